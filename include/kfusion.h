@@ -1,0 +1,241 @@
+/*
+ * kfusion.h — C ABI of the MI355X-native KinectFusion per-frame path.
+ *
+ * This is the drop-in boundary for the reference's per-frame path
+ * (isikmustafa/kinectfusion-cuda).  Every entry point takes plain pointers and
+ * sizes, returns an int status (KF_OK = 0) and never throws; the C++ class
+ * surface in include/kfusion/kfusion.hpp re-throws std::runtime_error exactly
+ * where the reference throws.  Citations are reference file:line.
+ *
+ * Conventions
+ *  - Device buffers are raw device pointers (allocated with kf_malloc or by any
+ *    other HIP allocator in the same process).
+ *  - Depth maps: float[h][w] metres.  Raw depth: uint16[h][w] codes.
+ *  - Vertex / normal maps: float4[h][w] (x, y, z, validity) with validity -1
+ *    valid and -2 invalid (project/src/device_helper.cuh:9-10).
+ *  - Matrices: glm column-major, m[col*rows + row] (test/example_test.cpp:42-51).
+ *  - Every stage takes a stream (NULL = the default stream) and an optional
+ *    float* elapsed_ms.  With elapsed_ms == NULL the call only enqueues work;
+ *    with elapsed_ms != NULL it records events around the work, waits for it and
+ *    stores the elapsed milliseconds — the reference's "every kernel wrapper
+ *    returns its time in ms" contract (project/src/cuda_event.cpp:14-30,
+ *    e.g. measurement.cu:139-158).
+ */
+#ifndef KFUSION_H
+#define KFUSION_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status --- */
+enum kf_status {
+    KF_OK = 0,
+    KF_ERR_DIMS = 1,      /* map sizes disagree (measurement.cu:144-147, raycast.cu:155-158, grid_map_pyramid.h:63-72) */
+    KF_ERR_NONFINITE = 2, /* ICP update not finite (icp.cpp:67-73) */
+    KF_ERR_NOT_PD = 3,    /* normal equations not positive definite (Eigen LLT NumericalIssue, ignored by the reference) */
+    KF_ERR_OOM = 4,
+    KF_ERR_HIP = 5,       /* HIP runtime error (the reference prints and continues, cuda_utils.h:7-23) */
+    KF_ERR_BUDGET = 6,    /* HBM budget exhausted (serving admission) */
+    KF_ERR_ARG = 7,       /* invalid argument */
+    KF_ERR_IO = 8         /* file I/O (voxel_grid.cpp:43-80 save/load) */
+};
+const char* kf_status_string(int status);
+/* Detail text of the last error on the calling thread ("" if none). */
+const char* kf_last_error(void);
+
+typedef void* kf_stream_t; /* a hipStream_t; NULL = default stream */
+
+/* ------------------------------------------------- device and memory plumbing */
+int kf_device_init(int device);
+int kf_device_synchronize(void);
+int kf_malloc(void** dptr, size_t bytes);
+int kf_free(void* dptr);
+int kf_memcpy_h2d(void* dst, const void* src, size_t bytes, kf_stream_t stream);
+int kf_memcpy_d2h(void* dst, const void* src, size_t bytes, kf_stream_t stream);
+int kf_memcpy_d2d(void* dst, const void* src, size_t bytes, kf_stream_t stream);
+int kf_stream_create(kf_stream_t* stream);
+int kf_stream_destroy(kf_stream_t stream);
+int kf_stream_synchronize(kf_stream_t stream);
+/* Events for timing on a given stream (cuda_event.h:5-17). */
+typedef void* kf_event_t;
+int kf_event_create(kf_event_t* ev);
+int kf_event_destroy(kf_event_t ev);
+int kf_event_record(kf_event_t ev, kf_stream_t stream);
+int kf_event_elapsed_ms(kf_event_t start, kf_event_t end, float* ms);
+
+/* ---------------------------------------------------------------- sensor --- */
+/* Sensor (project/src/sensor.h:7-53), passed by value to the kernels as in the
+ * reference.  Intrinsics per level: f_L = f / 2^L, c_L = (c_0 + 1/2) / 2^L - 1/2
+ * (sensor.h:14-16 hard-codes those values for 640x480). */
+typedef struct kf_sensor {
+    float pose[16];        /* camera -> world */
+    float inv_pose[16];    /* world -> camera */
+    float intr[3][9];      /* K_L */
+    float inv_intr[3][9];  /* K_L^-1 (glm::inverse, sensor.h:17-19) */
+} kf_sensor;
+/* Sensor(float fov) generalised to (fx, fy, cx, cy) (sensor.h:10-20); pose = I. */
+int kf_sensor_init(kf_sensor* s, float fx, float fy, float cx, float cy);
+/* Sensor::setPose (sensor.h:30-34); the inverse is [R^T | -R^T t] of the rigid pose. */
+int kf_sensor_set_pose(kf_sensor* s, const float pose[16]);
+
+/* ----------------------------------------------------------- measurement --- */
+/* kernel::convertToDepthMeters (measurement.cuh:12, measurement.cu:139-158). */
+int kf_convert_to_depth_meters(const uint16_t* raw, float* depth_m, int w, int h, float scale, int unmirror,
+                               kf_stream_t stream, float* elapsed_ms);
+/* kernel::applyBilateralFilter (measurement.cuh:13, measurement.cu:160-179). */
+int kf_apply_bilateral_filter(const float* depth_m, float* filtered, int w, int h, kf_stream_t stream,
+                              float* elapsed_ms);
+/* kernel::downSample (measurement.cuh:14, measurement.cu:181-200); the output must be half the input. */
+int kf_down_sample(const float* in, int w_in, int h_in, float* out, int w_out, int h_out, kf_stream_t stream,
+                   float* elapsed_ms);
+/* kernel::createVertexMap (measurement.cuh:15, measurement.cu:202-221). */
+int kf_create_vertex_map(const float* depth, float* vertex_map, int w, int h, const float inv_k[9],
+                         kf_stream_t stream, float* elapsed_ms);
+/* kernel::createNormalMap (measurement.cuh:16, measurement.cu:223-242). */
+int kf_create_normal_map(const float* vertex_map, float* normal_map, int w, int h, kf_stream_t stream,
+                         float* elapsed_ms);
+/* The whole measurement step of one frame in one launch (kinect_fusion.cpp:116-135
+ * plus the normal maps ICP uses): raw codes -> unfiltered metres (for fusion),
+ * filtered depth pyramid, vertex pyramid and normal pyramid, levels 0..levels-1.
+ * Results are identical to the individual stages above. */
+int kf_preprocess(const uint16_t* raw, int w, int h, float scale, int unmirror, int levels,
+                  const float inv_k[][9], float* depth_m, float* const depth_pyr[], float* const vertex_pyr[],
+                  float* const normal_pyr[], kf_stream_t stream, float* elapsed_ms);
+
+/* ----------------------------------------------------------- voxel grid --- */
+/* VoxelGrid / VoxelGridStruct / Voxel (voxel_grid.h:8-61).  The volume is an
+ * opaque handle; its HBM layout is 8^3 bricks of separate f and w planes. */
+typedef struct kf_volume kf_volume;
+enum kf_voxel_format {
+    KF_VOXEL_F32 = 0,        /* f: float, w: float (the reference's float2 Voxel) */
+    KF_VOXEL_SNORM16_U8 = 1  /* f: snorm16, w: uint8 observation count (cap 128) */
+};
+/* n must be a multiple of 8; resolution = width / (n - 1) (voxel_grid.h:43);
+ * mue_m is the truncation distance in metres (the reference passes
+ * mue = mue_scale * resolution, voxel_grid.h:44; see kf_volume_create_scaled). */
+int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** out);
+int kf_volume_create_scaled(int n, float width_m, float mue_scale, int format, kf_volume** out);
+int kf_volume_destroy(kf_volume* v);
+int kf_volume_info(const kf_volume* v, int* n, float* width_m, float* resolution, float* mue_m, int* format);
+/* Device bytes held by the volume (both planes). */
+int kf_volume_bytes(const kf_volume* v, size_t* bytes);
+/* Total number of voxels updated by every fuse into this volume so far. */
+int kf_volume_touched_total(const kf_volume* v, int64_t* count);
+/* kernel::initializeGrid (tsdf.cuh:15, tsdf.cu:79-90); value (f, w).  For the
+ * snorm16 format f is clamped to [-1, 1] and w rounded to an integer count. */
+int kf_initialize_grid(kf_volume* v, float f, float w, kf_stream_t stream, float* elapsed_ms);
+/* kernel::fuse (tsdf.cuh:16, tsdf.cu:92-103): projective TSDF update from an
+ * unfiltered metric depth map seen by the sensor.  The reference hard-codes a
+ * 640x480 frustum; here the frustum is the depth map's w x h. */
+int kf_fuse(const float* depth_m, int w, int h, kf_volume* v, const kf_sensor* sensor, kf_stream_t stream,
+            float* elapsed_ms);
+/* kernel::raycast (raycast.cuh:11, raycast.cu:149-169): world-space vertex and
+ * normal maps of level `level` seen from sensor.pose; pixels whose ray misses
+ * the volume or the [0.4, 6] m range are left untouched (raycast.cu:61-73). */
+int kf_raycast(const kf_volume* v, const kf_sensor* sensor, float* vertex_map, float* normal_map, int w, int h,
+               int level, kf_stream_t stream, float* elapsed_ms);
+/* VoxelGrid::saveVoxelGrid / loadVoxelGrid (voxel_grid.cpp:43-80): raw
+ * n^3 x (f, w) float pairs, x fastest, no header.  upload/download move the same
+ * layout between host memory and the volume. */
+int kf_volume_download(const kf_volume* v, float* fw_host);
+int kf_volume_upload(kf_volume* v, const float* fw_host);
+int kf_volume_save(const kf_volume* v, const char* path);
+int kf_volume_load(kf_volume* v, const char* path);
+
+/* ------------------------------------------------------------------ ICP --- */
+/* kernel::constructIcpResiduals (icp.cuh:34-37, icp.cu:86-106): one row
+ * A_i (6 floats, row-major N x 6) and b_i per pixel, zero rows when rejected. */
+int kf_construct_icp_residuals(const float* vertex_map, const float* target_vertex_map,
+                               const float* target_normal_map, int w, int h, const float prev_rot[9],
+                               const float prev_transl[3], const float cur_rot[9], const float cur_transl[3],
+                               const float intrinsics[9], float distance_thresh, float angle_thresh,
+                               float* mat_a, float* vec_b, kf_stream_t stream, float* elapsed_ms);
+/* LinearLeastSquares::solve (linear_least_squares.h:19, .cpp:23-44): solves the
+ * normal equations of A x = b for A (n x 6 row-major) and b on the device:
+ * a 27-term reduction (21 of A^T A, 6 of A^T b) and an FP64 Cholesky.
+ * result: 6 floats on the host.  KF_ERR_NOT_PD when A^T A is not positive definite. */
+int kf_solve_normal_equations(const float* mat_a, const float* vec_b, unsigned int n_equations, float result[6],
+                              kf_stream_t stream, float* elapsed_ms);
+/* One ICP iteration's association fused with its reduction: the 27 sums of the
+ * rows kf_construct_icp_residuals would produce, plus the number of accepted
+ * rows, without materialising A and b.  sums_host: 21 upper-triangle entries of
+ * A^T A in row order (p <= q), then A^T b; as doubles. */
+int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const float* target_normal_map,
+                  int w, int h, const float prev_rot[9], const float prev_transl[3], const float cur_rot[9],
+                  const float cur_transl[3], const float intrinsics[9], float distance_thresh, float angle_thresh,
+                  int compensated, double sums_host[27], int64_t* inliers, kf_stream_t stream, float* elapsed_ms);
+
+/* IcpConfig (icp.h:11-20). */
+typedef struct kf_icp_config {
+    int iters_per_layer[3];              /* index 0 = finest level */
+    int levels;                          /* 1..3 */
+    int width, height;                   /* level-0 map size */
+    float distance_thresh;               /* metres */
+    float angle_thresh;                  /* radians */
+    float iteration_stop_thresh_angle;   /* poseError chord (general_helper.cpp:11-21); 0 disables */
+    float iteration_stop_thresh_distance;/* metres; 0 disables */
+    int compensated;                     /* 0: FP32 tree sums; 1: Kahan-compensated FP32 sums */
+} kf_icp_config;
+typedef struct kf_icp kf_icp;
+/* ICP::ICP / ~ICP (icp.cpp:12-23). */
+int kf_icp_create(const kf_icp_config* config, kf_icp** out);
+int kf_icp_destroy(kf_icp* icp);
+/* ICP::computePose (icp.h:35-36, icp.cpp:25-58): coarse-to-fine pose of the
+ * current vertex pyramid against the predicted (raycast) pyramids, starting at
+ * sensor->pose.  The whole loop runs on the device; the call waits once at the
+ * end.  pose_out: camera -> world (16 floats).  KF_ERR_NONFINITE / KF_ERR_NOT_PD
+ * leave pose_out at the last good estimate. */
+int kf_icp_compute_pose(kf_icp* icp, const float* const vertex_pyr[], const float* const target_vertex_pyr[],
+                        const float* const target_normal_pyr[], const kf_sensor* sensor, float pose_out[16],
+                        kf_stream_t stream, float* elapsed_ms);
+/* ICP::getExecutionTimes (icp.h:38): {association+reduction ms, solve ms} of the last computePose. */
+int kf_icp_execution_times(const kf_icp* icp, float times[2]);
+/* Iterations executed by the last computePose (after early stops). */
+int kf_icp_iterations(const kf_icp* icp, int* iterations);
+
+/* ------------------------------------------------------------- per frame --- */
+/* KinectFusionConfig (kinect_fusion.h:17-36) restricted to the per-frame path:
+ * one volume serves tracking and fusion. */
+typedef struct kf_pipeline_config {
+    int frame_width, frame_height;
+    float depth_scale;          /* metres per raw code */
+    float fx, fy, cx, cy;       /* level-0 intrinsics */
+    int unmirror;               /* Kinect mirror flag of convertToDepthMeters */
+    int volume_n;
+    float volume_size;          /* cube side in metres, centred at the origin */
+    float mue;                  /* truncation distance in metres */
+    int voxel_format;           /* enum kf_voxel_format */
+    float initial_pose[16];     /* camera -> world of the first frame */
+    kf_icp_config icp;
+} kf_pipeline_config;
+typedef struct kf_pipeline kf_pipeline;
+enum kf_stage { KF_STAGE_PREPROCESS = 0, KF_STAGE_RAYCAST = 1, KF_STAGE_ICP = 2, KF_STAGE_FUSE = 3, KF_STAGE_COUNT = 4 };
+int kf_pipeline_create(const kf_pipeline_config* config, kf_stream_t stream, kf_pipeline** out);
+int kf_pipeline_destroy(kf_pipeline* p);
+/* KinectFusion::startTracking's loop body (kinect_fusion.cpp:48-72): the first
+ * frame is fused at the initial pose (initializeTracking, :75-94); later frames
+ * run preprocess -> raycast L0..L2 at the previous pose -> ICP -> fuse at the new
+ * pose.  raw: device pointer to one uint16 frame.  Enqueues only; frames whose
+ * ICP fails keep the previous pose and are not fused (reported by kf_pipeline_sync). */
+int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw);
+/* Waits for the enqueued frames; status: KF_OK or the first ICP failure since the last sync. */
+int kf_pipeline_sync(kf_pipeline* p, int* status);
+int kf_pipeline_pose(kf_pipeline* p, float pose[16]);
+int kf_pipeline_volume(kf_pipeline* p, kf_volume** v);
+/* which: 0 unfiltered metres (level 0), 1 filtered depth, 2 vertex, 3 normal,
+ *        4 predicted vertex, 5 predicted normal. */
+int kf_pipeline_map(kf_pipeline* p, int which, int level, void** dev_ptr, int* w, int* h);
+/* Per-stage event timing (kinect_fusion.cpp:216-224 prints these averages). */
+int kf_pipeline_set_timing(kf_pipeline* p, int enabled);
+int kf_pipeline_stage_times(kf_pipeline* p, float total_ms[KF_STAGE_COUNT], int64_t* frames);
+/* Average duration of the fuse launches timed since timing was enabled (ms). */
+int kf_pipeline_fuse_kernel_ms(kf_pipeline* p, float* avg_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KFUSION_H */

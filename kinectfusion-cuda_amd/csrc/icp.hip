@@ -1,0 +1,776 @@
+// Projective point-to-plane ICP (project/src/icp.cu, icp.cuh, icp.cpp,
+// linear_least_squares.cpp), kept on the device for a whole frame.
+//
+// One iteration = two launches:
+//   reduce   : 256 lanes x 4 pixels per workgroup.  Each lane associates its
+//              pixels (icp.cu:7-82) and accumulates the 27 unique terms of the
+//              normal equations (21 of A^T A, 6 of A^T b) in registers, either
+//              plainly in FP32 or compensated (TwoSum, Kahan-Babuska-Neumaier);
+//              then a 64-lane shuffle tree, an LDS pass over the 4 waves, and
+//              one partial row per workgroup.  A and b are never materialised.
+//   finalize : one workgroup sums the partial rows in a fixed order (so the
+//              result is deterministic), solves the 6x6 system with an FP64
+//              Cholesky, applies the pose update (icp.cpp:65-87) and the early
+//              stop test (icp.cpp:45-53) to the device-resident PoseState.
+// The reference instead writes A (N x 6) and b to HBM, runs cublasSgemm and
+// cublasSgemv, copies 42 floats to the host and solves with Eigen's FP32 LLT,
+// with a host synchronisation per iteration.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "kf_device.hpp"
+#include "kf_kernels.hpp"
+
+namespace kf {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPixPerThread = 4;
+constexpr int kTerms = 27;
+// partial layout (component-major): [0,27) sums, [27,54) compensations, 54 count
+constexpr int kCountSlot = 54;
+
+struct IcpArgs {
+    const float4* V;
+    const float4* TV;
+    const float4* TN;
+    int w, h;
+    float K[9];
+    float dist_thresh, cos_thresh;
+    PoseState* st;  // pose source (R, t, Rp, tp) and early-stop flags
+    int level;
+    float* partials;
+    int stride;     // blocks of this launch (partial row stride)
+};
+
+// icp.cu:7-82 for pixel (u, v); false = zero row.
+__device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const float* Rc, const float* tc,
+                                        const float* M, const float* tp, float* row, float& b)
+{
+    if (!map_valid(a.V, a.w, a.h, u, v)) return false;
+    v3 vc = xyz(a.V[(int64_t)v * a.w + u]);
+    v3 s = add3(m3v(Rc, vc), mk3(tc[0], tc[1], tc[2]));
+    // icp.cuh:40-49: (K * R_prev^T) * (s - t_prev), floor of the perspective division
+    v3 p = m3v(M, sub3(s, mk3(tp[0], tp[1], tp[2])));
+    float fu = floorf(p.x / p.z), fv = floorf(p.y / p.z);
+    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)a.w && fv < (float)a.h)) return false;
+    int uc = (int)fu, vcr = (int)fv;
+    if (!map_valid(a.TV, a.w, a.h, uc, vcr) || !map_valid(a.TN, a.w, a.h, uc, vcr)) return false;
+    v3 d = xyz(a.TV[(int64_t)vcr * a.w + uc]);
+    if (len3(sub3(d, s)) > a.dist_thresh) return false;  // icp.cuh:57-60
+    v3 n = xyz(a.TN[(int64_t)vcr * a.w + uc]);
+    float4 nl = compute_normal(a.V, a.w, a.h, u, v);
+    if (nl.w == kInvalid) return false;
+    // icp.cuh:63-70 in its cosine form: acos(d) > th  <=>  d >= -1 && d < cos(th)
+    v3 nn = normalize3(m3v(Rc, xyz(nl)));
+    float dd = dot3(nn, n);
+    if (dd >= -1.0f && dd < a.cos_thresh) return false;
+    // icp.cuh:72-95
+    row[0] = n.y * s.z - n.z * s.y;
+    row[1] = n.z * s.x - n.x * s.z;
+    row[2] = n.x * s.y - n.y * s.x;
+    row[3] = n.x;
+    row[4] = n.y;
+    row[5] = n.z;
+    b = ((((n.x * d.x + n.y * d.y) + n.z * d.z) - n.x * s.x) - n.y * s.y) - n.z * s.z;
+    return true;
+}
+
+// s += v with the rounding error of the addition carried in c (TwoSum).
+__device__ __forceinline__ void comp_add(float& s, float& c, float v)
+{
+    float t = s + v;
+    float bp = t - s;
+    float e = (s - (t - bp)) + (v - bp);
+    s = t;
+    c = c + e;
+}
+// (s1, c1) += (s2, c2)
+__device__ __forceinline__ void comp_merge(float& s1, float& c1, float s2, float c2)
+{
+    float t = s1 + s2;
+    float bp = t - s1;
+    float e = (s1 - (t - bp)) + (s2 - bp);
+    s1 = t;
+    c1 = (c1 + c2) + e;
+}
+
+template <bool COMP>
+__device__ __forceinline__ void accumulate(float* s, float* c, const float* row, float b)
+{
+    int q = 0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int r = p; r < 6; ++r, ++q) {
+            float v = row[p] * row[r];
+            if (COMP) comp_add(s[q], c[q], v); else s[q] = s[q] + v;
+        }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        float v = row[p] * b;
+        if (COMP) comp_add(s[21 + p], c[21 + p], v); else s[21 + p] = s[21 + p] + v;
+    }
+}
+
+// Reduces the lane values of a workgroup and writes one partial row.
+template <bool COMP>
+__device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, float* partials, int stride)
+{
+    __shared__ float red[kThreads / 64][2 * kTerms + 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < kTerms; ++q) {
+            float so = __shfl_xor(s[q], off, 64);
+            if (COMP) {
+                float co = __shfl_xor(c[q], off, 64);
+                comp_merge(s[q], c[q], so, co);
+            } else {
+                s[q] = s[q] + so;
+            }
+        }
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kTerms; ++q) {
+            red[wave][q] = s[q];
+            red[wave][kTerms + q] = COMP ? c[q] : 0.0f;
+        }
+        red[wave][2 * kTerms] = (float)cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x < kTerms) {
+        const int q = threadIdx.x;
+        float ss = red[0][q], cc = red[0][kTerms + q];
+        for (int wv = 1; wv < kThreads / 64; ++wv) {
+            if (COMP) comp_merge(ss, cc, red[wv][q], red[wv][kTerms + q]);
+            else ss = ss + red[wv][q];
+        }
+        partials[(int64_t)q * stride + blockIdx.x] = ss;
+        partials[(int64_t)(kTerms + q) * stride + blockIdx.x] = cc;
+    } else if (threadIdx.x == kTerms) {
+        float n = 0.0f;
+        for (int wv = 0; wv < kThreads / 64; ++wv) n = n + red[wv][2 * kTerms];
+        partials[(int64_t)kCountSlot * stride + blockIdx.x] = n;
+    }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_reduce_kernel(IcpArgs a)
+{
+    PoseState* st = a.st;
+    if (st->status != 0 || st->level_done[a.level]) return;
+    __shared__ float pose[24 + 9];
+    if (threadIdx.x < 9) pose[threadIdx.x] = st->R[threadIdx.x];
+    else if (threadIdx.x < 12) pose[threadIdx.x] = st->t[threadIdx.x - 9];
+    else if (threadIdx.x < 21) pose[threadIdx.x] = st->Rp[threadIdx.x - 12];
+    else if (threadIdx.x < 24) pose[threadIdx.x] = st->tp[threadIdx.x - 21];
+    __syncthreads();
+    const float* Rc = pose;
+    const float* tc = pose + 9;
+    float Rt[9], M[9];
+    m3t(pose + 12, Rt);
+    m3m3(a.K, Rt, M);
+    const float* tp = pose + 21;
+
+    float s[kTerms], c[kTerms];
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) s[q] = c[q] = 0.0f;
+    int cnt = 0;
+    const int64_t npx = (int64_t)a.w * a.h;
+#pragma unroll
+    for (int k = 0; k < kPixPerThread; ++k) {
+        int64_t px = ((int64_t)blockIdx.x * kPixPerThread + k) * kThreads + threadIdx.x;
+        if (px >= npx) break;
+        int u = (int)(px % a.w), v = (int)(px / a.w);
+        float row[6], b;
+        if (icp_row(a, u, v, Rc, tc, M, tp, row, b)) {
+            accumulate<COMP>(s, c, row, b);
+            ++cnt;
+        }
+    }
+    block_reduce_store<COMP>(s, c, cnt, a.partials, a.stride);
+}
+
+// Rows of a given A (n x 6 row-major) and b (LinearLeastSquares::solve).
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void ab_reduce_kernel(const float* A, const float* B, int64_t n,
+                                                             float* partials, int stride)
+{
+    float s[kTerms], c[kTerms];
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) s[q] = c[q] = 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kPixPerThread; ++k) {
+        int64_t r = ((int64_t)blockIdx.x * kPixPerThread + k) * kThreads + threadIdx.x;
+        if (r >= n) break;
+        float row[6];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) row[p] = A[r * 6 + p];
+        float b = B[r];
+        accumulate<COMP>(s, c, row, b);
+        bool nz = b != 0.0f;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) nz = nz || row[p] != 0.0f;
+        cnt += nz ? 1 : 0;
+    }
+    block_reduce_store<COMP>(s, c, cnt, partials, stride);
+}
+
+// icp.cu:7-82, writing the rows (kernel::constructIcpResiduals).
+__global__ __launch_bounds__(kThreads) void icp_rows_kernel(IcpArgs a, float* A, float* B)
+{
+    const float* pose = reinterpret_cast<const float*>(a.partials);  // 24 floats by value (R, t, Rp, tp)
+    float Rt[9], M[9];
+    m3t(pose + 12, Rt);
+    m3m3(a.K, Rt, M);
+    int64_t px = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (px >= (int64_t)a.w * a.h) return;
+    int u = (int)(px % a.w), v = (int)(px / a.w);
+    float row[6] = {0, 0, 0, 0, 0, 0}, b = 0.0f;
+    if (!icp_row(a, u, v, pose, pose + 9, M, pose + 21, row, b)) {
+#pragma unroll
+        for (int p = 0; p < 6; ++p) row[p] = 0.0f;
+        b = 0.0f;
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) A[px * 6 + p] = row[p];
+    B[px] = b;
+}
+
+// ---- FP64 6x6 Cholesky, unblocked, in the column order of Eigen's LLT
+// (3rd/eigen/Eigen/src/Cholesky/LLT.h:306-326); 0 or the failing column + 1.
+__device__ int chol6(const double* ata, const double* atb, double* x)
+{
+    double L[6][6];
+    for (int c = 0; c < 6; ++c)
+        for (int r = 0; r < 6; ++r) L[r][c] = ata[c * 6 + r];
+    for (int k = 0; k < 6; ++k) {
+        double d = L[k][k];
+        for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
+        if (!(d > 0.0)) return k + 1;
+        d = sqrt(d);
+        L[k][k] = d;
+        for (int r = k + 1; r < 6; ++r) {
+            double v = L[r][k];
+            for (int j = 0; j < k; ++j) v -= L[r][j] * L[k][j];
+            L[r][k] = v / d;
+        }
+    }
+    double y[6];
+    for (int r = 0; r < 6; ++r) {
+        double v = atb[r];
+        for (int j = 0; j < r; ++j) v -= L[r][j] * y[j];
+        y[r] = v / L[r][r];
+    }
+    for (int r = 5; r >= 0; --r) {
+        double v = y[r];
+        for (int j = r + 1; j < 6; ++j) v -= L[j][r] * x[j];
+        x[r] = v / L[r][r];
+    }
+    return 0;
+}
+
+// icp.cpp:65-87 in FP64: R_inc = R_y(x1) R_x(x0) R_z(x2) (glm orientate3(x0, x2, x1)),
+// R <- R_inc^T R, t <- R_inc^T t + x[3..5].
+__device__ void update_pose(float* Rm, float* t, const float* x)
+{
+    double pitch = x[0], yaw = x[1], roll = x[2];
+    double ch = cos(yaw), sh = sin(yaw), cp = cos(pitch), sp = sin(pitch), cb = cos(roll), sb = sin(roll);
+    double Ri[9];
+    Ri[0] = ch * cb + sh * sp * sb;  Ri[1] = sb * cp;  Ri[2] = -sh * cb + ch * sp * sb;
+    Ri[3] = -ch * sb + sh * sp * cb; Ri[4] = cb * cp;  Ri[5] = sb * sh + ch * sp * cb;
+    Ri[6] = sh * cp;                 Ri[7] = -sp;      Ri[8] = ch * cp;
+    double Rn[9], tn[3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+            for (int q = 0; q < 3; ++q) s += Ri[r * 3 + q] * (double)Rm[c * 3 + q];
+            Rn[c * 3 + r] = s;
+        }
+    for (int r = 0; r < 3; ++r) {
+        double s = 0;
+        for (int q = 0; q < 3; ++q) s += Ri[r * 3 + q] * (double)t[q];
+        tn[r] = s + (double)x[3 + r];
+    }
+    for (int q = 0; q < 9; ++q) Rm[q] = (float)Rn[q];
+    for (int q = 0; q < 3; ++q) t[q] = (float)tn[q];
+}
+
+// general_helper.cpp:11-21
+__device__ void pose_error(const float* R1, const float* t1, const float* R2, const float* t2, float* out)
+{
+    v3 v = normalize3(mk3(1.0f, 1.0f, 1.0f));
+    v3 a = m3v(R1, v), b = m3v(R2, v);
+    out[0] = len3(sub3(b, a));
+    out[1] = len3(sub3(mk3(t2[0], t2[1], t2[2]), mk3(t1[0], t1[1], t1[2])));
+}
+
+// Sums the partial rows (fixed order) into 27 doubles + count in LDS.
+template <bool COMP>
+__device__ void sum_partials(const float* partials, int stride, int nblocks, double* tot, double* count)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int q = wave; q <= kTerms; q += kThreads / 64) {
+        const bool is_count = (q == kTerms);
+        const float* sp = partials + (int64_t)(is_count ? kCountSlot : q) * stride;
+        const float* cp = partials + (int64_t)(kTerms + q) * stride;
+        float s = 0.0f, c = 0.0f;
+        for (int b = lane; b < nblocks; b += 64) {
+            if (COMP && !is_count) comp_merge(s, c, sp[b], cp[b]);
+            else s = s + sp[b];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            float so = __shfl_xor(s, off, 64);
+            if (COMP && !is_count) {
+                float co = __shfl_xor(c, off, 64);
+                comp_merge(s, c, so, co);
+            } else {
+                s = s + so;
+            }
+        }
+        if (lane == 0) {
+            if (is_count) *count = (double)s;
+            else tot[q] = COMP ? (double)s + (double)c : (double)s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void expand(const double* tot, double* ata, double* atb)
+{
+    int q = 0;
+    for (int p = 0; p < 6; ++p)
+        for (int r = p; r < 6; ++r, ++q) {
+            ata[r * 6 + p] = tot[q];
+            ata[p * 6 + r] = tot[q];
+        }
+    for (int p = 0; p < 6; ++p) atb[p] = tot[21 + p];
+}
+
+// mode 0: ICP iteration (solve + pose update + early stop); mode 1: solve only.
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, const float* partials, int stride,
+                                                                int nblocks, int level, float stop_angle,
+                                                                float stop_dist, int mode)
+{
+    if (mode == 0 && (st->status != 0 || st->level_done[level])) return;
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    sum_partials<COMP>(partials, stride, nblocks, tot, &count);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < kTerms; ++q) st->last_sums[q] = tot[q];
+    st->last_inliers = (long long)count;
+    double ata[36], atb[6], xd[6];
+    expand(tot, ata, atb);
+    if (chol6(ata, atb, xd) != 0) {
+        if (st->status == 0) st->status = KF_ERR_NOT_PD;
+        if (st->sticky_status == 0) st->sticky_status = KF_ERR_NOT_PD;
+        st->frame_skip_fuse = 1;
+        return;
+    }
+    float x[6];
+    bool finite = true;
+    for (int q = 0; q < 6; ++q) {
+        x[q] = (float)xd[q];
+        finite = finite && isfinite(x[q]);
+        st->last_x[q] = x[q];
+    }
+    if (!finite) {
+        if (st->status == 0) st->status = KF_ERR_NONFINITE;
+        if (st->sticky_status == 0) st->sticky_status = KF_ERR_NONFINITE;
+        st->frame_skip_fuse = 1;
+        return;
+    }
+    if (mode != 0) return;
+    float Rprev[9], tprev[3], err[2];
+    for (int q = 0; q < 9; ++q) Rprev[q] = st->R[q];
+    for (int q = 0; q < 3; ++q) tprev[q] = st->t[q];
+    update_pose(st->R, st->t, x);
+    pose_error(Rprev, tprev, st->R, st->t, err);
+    st->iterations = st->iterations + 1;
+    if (err[0] < stop_angle && err[1] < stop_dist) st->level_done[level] = 1;
+}
+
+// Starts a frame's ICP: the working and previous poses are the sensor pose
+// (icp.cpp:27-29); from `pose` when given by value, else from st->pose.
+struct Pose16 {
+    float m[16];
+};
+__global__ void icp_begin_kernel(PoseState* st, Pose16 pose, int from_value)
+{
+    if (threadIdx.x != 0) return;
+    if (from_value)
+        for (int q = 0; q < 16; ++q) st->pose[q] = pose.m[q];
+    const float* P = st->pose;
+    float Rm[9] = {P[0], P[1], P[2], P[4], P[5], P[6], P[8], P[9], P[10]};
+    for (int q = 0; q < 9; ++q) st->R[q] = st->Rp[q] = Rm[q];
+    for (int q = 0; q < 3; ++q) st->t[q] = st->tp[q] = P[12 + q];
+    st->status = 0;
+    for (int l = 0; l < 4; ++l) st->level_done[l] = 0;
+    st->iterations = 0;
+    st->frame_skip_fuse = 0;
+}
+
+// Sensor::setPose of the estimate (kinect_fusion.cpp:146-161, sensor.h:30-34).
+__global__ void icp_commit_kernel(PoseState* st)
+{
+    if (threadIdx.x != 0 || st->status != 0) return;
+    float* P = st->pose;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) P[c * 4 + r] = st->R[c * 3 + r];
+    P[3] = P[7] = P[11] = 0.0f;
+    for (int r = 0; r < 3; ++r) P[12 + r] = st->t[r];
+    P[15] = 1.0f;
+    float* I = st->inv_pose;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) I[c * 4 + r] = st->R[r * 3 + c];
+    I[3] = I[7] = I[11] = 0.0f;
+    for (int r = 0; r < 3; ++r) {
+        double s = 0;
+        for (int q = 0; q < 3; ++q) s += (double)st->R[r * 3 + q] * (double)st->t[q];
+        I[12 + r] = (float)(-s);
+    }
+    I[15] = 1.0f;
+}
+
+inline int blocks_for(int64_t n) { return (int)((n + kThreads * kPixPerThread - 1) / (kThreads * kPixPerThread)); }
+
+}  // namespace
+
+int icp_blocks_for(int w, int h) { return blocks_for((int64_t)w * h); }
+
+int launch_icp_begin(PoseState* st, const float* pose16, hipStream_t s)
+{
+    Pose16 p{};
+    if (pose16) memcpy(p.m, pose16, sizeof(p.m));
+    icp_begin_kernel<<<1, 64, 0, s>>>(st, p, pose16 ? 1 : 0);
+    KF_LAUNCHED("icp_begin_kernel");
+    return KF_OK;
+}
+
+int launch_icp_commit(PoseState* st, hipStream_t s)
+{
+    icp_commit_kernel<<<1, 64, 0, s>>>(st);
+    KF_LAUNCHED("icp_commit_kernel");
+    return KF_OK;
+}
+
+static int launch_iteration(kf_icp* icp, const IcpLevel& L, int level, hipStream_t s, hipEvent_t mid)
+{
+    IcpArgs a{};
+    a.V = L.V;
+    a.TV = L.TV;
+    a.TN = L.TN;
+    a.w = L.w;
+    a.h = L.h;
+    memcpy(a.K, L.K, sizeof(a.K));
+    a.dist_thresh = icp->cfg.distance_thresh;
+    a.cos_thresh = icp->cos_thresh;
+    a.st = icp->state;
+    a.level = level;
+    a.partials = icp->partials;
+    const int blocks = blocks_for((int64_t)L.w * L.h);
+    a.stride = blocks;
+    if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
+    if (icp->cfg.compensated) icp_reduce_kernel<true><<<blocks, kThreads, 0, s>>>(a);
+    else icp_reduce_kernel<false><<<blocks, kThreads, 0, s>>>(a);
+    KF_LAUNCHED("icp_reduce_kernel");
+    if (mid) KF_HIP(hipEventRecord(mid, s));
+    const float sa = icp->cfg.iteration_stop_thresh_angle, sd = icp->cfg.iteration_stop_thresh_distance;
+    if (icp->cfg.compensated)
+        icp_finalize_kernel<true><<<1, kThreads, 0, s>>>(icp->state, icp->partials, blocks, blocks, level, sa, sd, 0);
+    else
+        icp_finalize_kernel<false><<<1, kThreads, 0, s>>>(icp->state, icp->partials, blocks, blocks, level, sa, sd, 0);
+    KF_LAUNCHED("icp_finalize_kernel");
+    return KF_OK;
+}
+
+int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s)
+{
+    for (int l = n_levels - 1; l >= 0; --l)
+        for (int it = 0; it < icp->cfg.iters_per_layer[l]; ++it) {
+            int st = launch_iteration(icp, levels[l], l, s, nullptr);
+            if (st) return st;
+        }
+    return KF_OK;
+}
+
+}  // namespace kf
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int kf_construct_icp_residuals(const float* vertex_map, const float* target_vertex_map, const float* target_normal_map,
+                               int w, int h, const float prev_rot[9], const float prev_transl[3],
+                               const float cur_rot[9], const float cur_transl[3], const float intrinsics[9],
+                               float distance_thresh, float angle_thresh, float* mat_a, float* vec_b,
+                               kf_stream_t stream, float* elapsed_ms)
+{
+    if (!vertex_map || !target_vertex_map || !target_normal_map || !mat_a || !vec_b || w <= 0 || h <= 0 || !prev_rot ||
+        !prev_transl || !cur_rot || !cur_transl || !intrinsics)
+        return kf::fail(KF_ERR_ARG, "constructIcpResiduals: bad argument");
+    hipStream_t s = kf::stream_of(stream);
+    // the pose travels by value through a 24-float staging block
+    float pose[24];
+    memcpy(pose, cur_rot, 9 * sizeof(float));
+    memcpy(pose + 9, cur_transl, 3 * sizeof(float));
+    memcpy(pose + 12, prev_rot, 9 * sizeof(float));
+    memcpy(pose + 21, prev_transl, 3 * sizeof(float));
+    float* dpose = nullptr;
+    KF_HIP(hipMalloc(&dpose, sizeof(pose)));
+    kf::IcpArgs a{};
+    a.V = reinterpret_cast<const float4*>(vertex_map);
+    a.TV = reinterpret_cast<const float4*>(target_vertex_map);
+    a.TN = reinterpret_cast<const float4*>(target_normal_map);
+    a.w = w;
+    a.h = h;
+    memcpy(a.K, intrinsics, sizeof(a.K));
+    a.dist_thresh = distance_thresh;
+    a.cos_thresh = (float)cos((double)angle_thresh);
+    a.partials = dpose;
+    kf::CallTimer t(elapsed_ms, s);
+    int st = KF_OK;
+    hipError_t e = hipMemcpyAsync(dpose, pose, sizeof(pose), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) st = kf::fail_hip(e, "constructIcpResiduals pose copy");
+    if (st == KF_OK) st = t.start();
+    if (st == KF_OK) {
+        const unsigned blocks = (unsigned)(((int64_t)w * h + kf::kThreads - 1) / kf::kThreads);
+        kf::icp_rows_kernel<<<blocks, kf::kThreads, 0, s>>>(a, mat_a, vec_b);
+        e = hipGetLastError();
+        if (e != hipSuccess) st = kf::fail_hip(e, "icp_rows_kernel");
+    }
+    if (st == KF_OK) st = t.finish();
+    e = hipStreamSynchronize(s);
+    if (st == KF_OK && e != hipSuccess) st = kf::fail_hip(e, "constructIcpResiduals");
+    (void)hipFree(dpose);
+    return st;
+}
+
+// Shared by kf_solve_normal_equations and kf_icp_reduce: partial rows -> sums/solve.
+static int reduce_to_state(kf::PoseState** st_out, float** partials_out, int blocks)
+{
+    KF_HIP(hipMalloc(st_out, sizeof(kf::PoseState)));
+    KF_HIP(hipMemset(*st_out, 0, sizeof(kf::PoseState)));
+    KF_HIP(hipMalloc(partials_out, sizeof(float) * kf::kPartialStride * (size_t)(blocks > 0 ? blocks : 1)));
+    return KF_OK;
+}
+
+int kf_solve_normal_equations(const float* mat_a, const float* vec_b, unsigned int n_equations, float result[6],
+                              kf_stream_t stream, float* elapsed_ms)
+{
+    if (!mat_a || !vec_b || !result || n_equations == 0)
+        return kf::fail(KF_ERR_ARG, "LinearLeastSquares::solve: bad argument");
+    hipStream_t s = kf::stream_of(stream);
+    const int blocks = kf::blocks_for(n_equations);
+    kf::PoseState* st = nullptr;
+    float* partials = nullptr;
+    int rc = reduce_to_state(&st, &partials, blocks);
+    kf::CallTimer t(elapsed_ms, s);
+    if (rc == KF_OK) rc = t.start();
+    if (rc == KF_OK) {
+        kf::ab_reduce_kernel<false><<<blocks, kf::kThreads, 0, s>>>(mat_a, vec_b, n_equations, partials, blocks);
+        kf::icp_finalize_kernel<false><<<1, kf::kThreads, 0, s>>>(st, partials, blocks, blocks, 0, 0.0f, 0.0f, 1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = kf::fail_hip(e, "solve kernels");
+    }
+    if (rc == KF_OK) rc = t.finish();
+    kf::PoseState hs;
+    if (rc == KF_OK) {
+        hipError_t e = hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = kf::fail_hip(e, "solve readback");
+    }
+    if (rc == KF_OK) {
+        for (int q = 0; q < 6; ++q) result[q] = hs.last_x[q];
+        if (hs.status) rc = kf::fail(hs.status, kf_status_string(hs.status));
+    }
+    (void)hipFree(st);
+    (void)hipFree(partials);
+    return rc;
+}
+
+int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const float* target_normal_map, int w,
+                  int h, const float prev_rot[9], const float prev_transl[3], const float cur_rot[9],
+                  const float cur_transl[3], const float intrinsics[9], float distance_thresh, float angle_thresh,
+                  int compensated, double sums_host[27], int64_t* inliers, kf_stream_t stream, float* elapsed_ms)
+{
+    if (!vertex_map || !target_vertex_map || !target_normal_map || !sums_host || w <= 0 || h <= 0)
+        return kf::fail(KF_ERR_ARG, "kf_icp_reduce: bad argument");
+    hipStream_t s = kf::stream_of(stream);
+    const int blocks = kf::blocks_for((int64_t)w * h);
+    kf::PoseState* st = nullptr;
+    float* partials = nullptr;
+    int rc = reduce_to_state(&st, &partials, blocks);
+    kf::PoseState hs;
+    memset(&hs, 0, sizeof(hs));
+    memcpy(hs.R, cur_rot, sizeof(hs.R));
+    memcpy(hs.t, cur_transl, sizeof(hs.t));
+    memcpy(hs.Rp, prev_rot, sizeof(hs.Rp));
+    memcpy(hs.tp, prev_transl, sizeof(hs.tp));
+    if (rc == KF_OK) {
+        hipError_t e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = kf::fail_hip(e, "kf_icp_reduce state");
+    }
+    kf::IcpArgs a{};
+    a.V = reinterpret_cast<const float4*>(vertex_map);
+    a.TV = reinterpret_cast<const float4*>(target_vertex_map);
+    a.TN = reinterpret_cast<const float4*>(target_normal_map);
+    a.w = w;
+    a.h = h;
+    memcpy(a.K, intrinsics, sizeof(a.K));
+    a.dist_thresh = distance_thresh;
+    a.cos_thresh = (float)cos((double)angle_thresh);
+    a.st = st;
+    a.level = 0;
+    a.partials = partials;
+    a.stride = blocks;
+    kf::CallTimer t(elapsed_ms, s);
+    if (rc == KF_OK) rc = t.start();
+    if (rc == KF_OK) {
+        if (compensated) {
+            kf::icp_reduce_kernel<true><<<blocks, kf::kThreads, 0, s>>>(a);
+            kf::icp_finalize_kernel<true><<<1, kf::kThreads, 0, s>>>(st, partials, blocks, blocks, 0, 0, 0, 1);
+        } else {
+            kf::icp_reduce_kernel<false><<<blocks, kf::kThreads, 0, s>>>(a);
+            kf::icp_finalize_kernel<false><<<1, kf::kThreads, 0, s>>>(st, partials, blocks, blocks, 0, 0, 0, 1);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = kf::fail_hip(e, "kf_icp_reduce kernels");
+    }
+    if (rc == KF_OK) rc = t.finish();
+    if (rc == KF_OK) {
+        hipError_t e = hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = kf::fail_hip(e, "kf_icp_reduce readback");
+    }
+    if (rc == KF_OK) {
+        for (int q = 0; q < 27; ++q) sums_host[q] = hs.last_sums[q];
+        if (inliers) *inliers = hs.last_inliers;
+    }
+    (void)hipFree(st);
+    (void)hipFree(partials);
+    return rc;
+}
+
+int kf_icp_create(const kf_icp_config* config, kf_icp** out)
+{
+    if (!config || !out) return kf::fail(KF_ERR_ARG, "kf_icp_create: null argument");
+    *out = nullptr;
+    if (config->levels < 1 || config->levels > 3 || config->width <= 0 || config->height <= 0)
+        return kf::fail(KF_ERR_ARG, "kf_icp_create: levels must be 1..3 and sizes positive");
+    for (int l = 0; l < config->levels; ++l)
+        if (config->iters_per_layer[l] < 0) return kf::fail(KF_ERR_ARG, "kf_icp_create: negative iteration count");
+    kf_icp* icp = new kf_icp();
+    icp->cfg = *config;
+    icp->cos_thresh = (float)cos((double)config->angle_thresh);
+    icp->max_blocks = kf::icp_blocks_for(config->width, config->height);
+    hipError_t e;
+    if ((e = hipMalloc(&icp->state, sizeof(kf::PoseState))) != hipSuccess ||
+        (e = hipMemset(icp->state, 0, sizeof(kf::PoseState))) != hipSuccess ||
+        (e = hipMalloc(&icp->partials, sizeof(float) * kf::kPartialStride * (size_t)icp->max_blocks)) != hipSuccess) {
+        kf_icp_destroy(icp);
+        return kf::fail_hip(e, "kf_icp_create");
+    }
+    *out = icp;
+    return KF_OK;
+}
+
+int kf_icp_destroy(kf_icp* icp)
+{
+    if (!icp) return KF_OK;
+    (void)hipFree(icp->state);
+    (void)hipFree(icp->partials);
+    delete icp;
+    return KF_OK;
+}
+
+int kf_icp_compute_pose(kf_icp* icp, const float* const vertex_pyr[], const float* const target_vertex_pyr[],
+                        const float* const target_normal_pyr[], const kf_sensor* sensor, float pose_out[16],
+                        kf_stream_t stream, float* elapsed_ms)
+{
+    if (!icp || !vertex_pyr || !target_vertex_pyr || !target_normal_pyr || !sensor || !pose_out)
+        return kf::fail(KF_ERR_ARG, "ICP::computePose: null argument");
+    hipStream_t s = kf::stream_of(stream);
+    kf::IcpLevel lv[3];
+    for (int l = 0; l < icp->cfg.levels; ++l) {
+        lv[l].V = reinterpret_cast<const float4*>(vertex_pyr[l]);
+        lv[l].TV = reinterpret_cast<const float4*>(target_vertex_pyr[l]);
+        lv[l].TN = reinterpret_cast<const float4*>(target_normal_pyr[l]);
+        lv[l].w = icp->cfg.width >> l;
+        lv[l].h = icp->cfg.height >> l;
+        memcpy(lv[l].K, sensor->intr[l], sizeof(lv[l].K));
+    }
+    // Events around every association+reduction and every solve (ICP::getExecutionTimes).
+    int total = 0;
+    for (int l = 0; l < icp->cfg.levels; ++l) total += icp->cfg.iters_per_layer[l];
+    std::vector<hipEvent_t> ev((size_t)total * 3 + 2, nullptr);
+    int rc = KF_OK;
+    for (auto& e : ev)
+        if (rc == KF_OK && hipEventCreate(&e) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event create");
+    if (rc == KF_OK) rc = kf::launch_icp_begin(icp->state, sensor->pose, s);
+    if (rc == KF_OK && hipEventRecord(ev[0], s) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event record");
+    int k = 0;
+    for (int l = icp->cfg.levels - 1; l >= 0 && rc == KF_OK; --l)
+        for (int it = 0; it < icp->cfg.iters_per_layer[l] && rc == KF_OK; ++it, ++k) {
+            (void)hipEventRecord(ev[1 + 3 * k], s);
+            rc = kf::launch_iteration(icp, lv[l], l, s, ev[2 + 3 * k]);
+            (void)hipEventRecord(ev[3 + 3 * k], s);
+        }
+    if (rc == KF_OK) rc = kf::launch_icp_commit(icp->state, s);
+    if (rc == KF_OK && hipEventRecord(ev.back(), s) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event record");
+    kf::PoseState hs;
+    if (rc == KF_OK) {
+        hipError_t e = hipMemcpyAsync(&hs, icp->state, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = kf::fail_hip(e, "ICP readback");
+    }
+    if (rc == KF_OK) {
+        icp->times[0] = icp->times[1] = 0.0f;
+        for (int q = 0; q < k; ++q) {
+            float a = 0, b = 0;
+            (void)hipEventElapsedTime(&a, ev[1 + 3 * q], ev[2 + 3 * q]);
+            (void)hipEventElapsedTime(&b, ev[2 + 3 * q], ev[3 + 3 * q]);
+            icp->times[0] += a;
+            icp->times[1] += b;
+        }
+        if (elapsed_ms) (void)hipEventElapsedTime(elapsed_ms, ev[0], ev.back());
+        icp->last_iterations = hs.iterations;
+        // the last good estimate (the commit does not run on failure)
+        float P[16] = {0};
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) P[c * 4 + r] = hs.R[c * 3 + r];
+        for (int r = 0; r < 3; ++r) P[12 + r] = hs.t[r];
+        P[15] = 1.0f;
+        memcpy(pose_out, P, sizeof(P));
+        if (hs.status) rc = kf::fail(hs.status, kf_status_string(hs.status));
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int kf_icp_execution_times(const kf_icp* icp, float times[2])
+{
+    if (!icp || !times) return kf::fail(KF_ERR_ARG, "kf_icp_execution_times: null argument");
+    times[0] = icp->times[0];
+    times[1] = icp->times[1];
+    return KF_OK;
+}
+
+int kf_icp_iterations(const kf_icp* icp, int* iterations)
+{
+    if (!icp || !iterations) return kf::fail(KF_ERR_ARG, "kf_icp_iterations: null argument");
+    *iterations = icp->last_iterations;
+    return KF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// Device-side math shared by every kernel of the per-frame path.
+//
+// Every expression below is evaluated in a fixed operation order with no FMA
+// contraction (the library is compiled with -ffp-contract=off) and with
+// correctly rounded FP32 division and square root (hipcc's default for HIP
+// device code), so a kernel's result is a deterministic function of its inputs.
+// The orders are glm's, as the reference uses them (project/src/*.cu include
+// glm with GLM_FORCE_PURE, project/src/glm_macro.h:5-12):
+//   mat3*vec3 : r_i = (m[0][i]*v.x + m[1][i]*v.y) + m[2][i]*v.z
+//   mat4*vec4 : r   = (m[0]*v.x + m[1]*v.y) + (m[2]*v.z + m[3]*v.w)
+//   mat3*mat3 : C[c][r] = (A[0][r]*B[c][0] + A[1][r]*B[c][1]) + A[2][r]*B[c][2]
+//   dot       : (a.x*b.x + a.y*b.y) + a.z*b.z
+//   normalize : v * (1 / sqrt(dot(v, v)))
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kf {
+
+constexpr float kValid = -1.0f;     // device_helper.cuh:9
+constexpr float kInvalid = -2.0f;   // device_helper.cuh:10
+constexpr float kMinDepth = 0.01f;  // device_helper.cuh:11
+constexpr float kMinDistance = 0.4f;  // tsdf.cuh:11
+constexpr float kMaxDistance = 6.0f;  // tsdf.cuh:12
+
+struct v3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 add3(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 sub3(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 mul3s(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 div3s(v3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float len3(v3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ v3 normalize3(v3 a)
+{
+    float inv = 1.0f / sqrtf(dot3(a, a));
+    return mul3s(a, inv);
+}
+__device__ __forceinline__ v3 cross3(v3 x, v3 y)
+{
+    return mk3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
+}
+// m: column-major 3x3
+__device__ __forceinline__ v3 m3v(const float* m, v3 v)
+{
+    return mk3((m[0] * v.x + m[3] * v.y) + m[6] * v.z,
+               (m[1] * v.x + m[4] * v.y) + m[7] * v.z,
+               (m[2] * v.x + m[5] * v.y) + m[8] * v.z);
+}
+__device__ __forceinline__ void m3m3(const float* A, const float* B, float* C)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            C[c * 3 + r] = (A[0 * 3 + r] * B[c * 3 + 0] + A[1 * 3 + r] * B[c * 3 + 1]) + A[2 * 3 + r] * B[c * 3 + 2];
+}
+__device__ __forceinline__ void m3t(const float* A, float* T)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) T[c * 3 + r] = A[r * 3 + c];
+}
+// glm mat4 * vec4(p, 1), xyz only
+__device__ __forceinline__ v3 m4p(const float* m, v3 p)
+{
+    float a0 = m[0] * p.x, a1 = m[4] * p.y, a2 = m[8] * p.z, a3 = m[12] * 1.0f;
+    float b0 = m[1] * p.x, b1 = m[5] * p.y, b2 = m[9] * p.z, b3 = m[13] * 1.0f;
+    float c0 = m[2] * p.x, c1 = m[6] * p.y, c2 = m[10] * p.z, c3 = m[14] * 1.0f;
+    return mk3((a0 + a1) + (a2 + a3), (b0 + b1) + (b2 + b3), (c0 + c1) + (c2 + c3));
+}
+
+__device__ __forceinline__ bool is_depth_valid(float d) { return d > kMinDepth && isfinite(d); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// float4 map access; out-of-range validity reads are invalid (device_helper.cuh:43-50)
+__device__ __forceinline__ bool map_valid(const float4* m, int w, int h, int i, int j)
+{
+    if (i < 0 || j < 0 || i >= w || j >= h) return false;
+    return m[(int64_t)j * w + i].w == kValid;
+}
+__device__ __forceinline__ v3 xyz(float4 e) { return mk3(e.x, e.y, e.z); }
+
+// Raw depth code -> binary16 (RNE) -> float, integer restatement of the IEEE
+// rule (measurement.cu:19-21); codes >= 65520 overflow to +inf.
+__device__ __forceinline__ float half_rne(uint32_t c)
+{
+    if (c <= 2048u) return (float)c;
+    int bits = 32 - __clz(c);
+    int drop = bits - 11;
+    uint32_t q = c >> drop, r = c & ((1u << drop) - 1u), half = 1u << (drop - 1);
+    if (r > half || (r == half && (q & 1u))) q += 1u;
+    uint32_t v = q << drop;
+    if (v > 65504u) return __int_as_float(0x7f800000);
+    return (float)v;
+}
+
+// Single-precision exp: Cody-Waite reduction by ln2 with the Cephes expf
+// polynomial (S. L. Moshier), then exact scaling by 2^k with one final rounding.
+__device__ __forceinline__ float kf_expf(float x)
+{
+    if (!(x == x)) return x;
+    if (x < -104.0f) return 0.0f;
+    if (x > 88.72283f) return __int_as_float(0x7f800000);
+    float fk = floorf(x * 1.44269504088896341f + 0.5f);
+    int k = (int)fk;
+    float r = x - fk * 0.693359375f;
+    r = r - fk * -2.12194440e-4f;
+    float z = r * r;
+    float p = 1.9875691500e-4f;
+    p = p * r + 1.3981999507e-3f;
+    p = p * r + 8.3334519073e-3f;
+    p = p * r + 4.1665795894e-2f;
+    p = p * r + 1.6666665459e-1f;
+    p = p * r + 5.0000001201e-1f;
+    float y = p * z + r;
+    y = y + 1.0f;
+    if (k > 127) { y = y * 1.7014118346046923e38f; k -= 127; }
+    if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
+    return y * __int_as_float((k + 127) << 23);
+}
+
+// Live normal at (i,j) of a vertex map (device_helper.cuh:81-110).
+__device__ __forceinline__ float4 compute_normal(const float4* v, int w, int h, int i, int j)
+{
+    if (map_valid(v, w, h, i, j) && map_valid(v, w, h, i + 1, j) && map_valid(v, w, h, i, j + 1)) {
+        v3 c = xyz(v[(int64_t)j * w + i]);
+        v3 r = xyz(v[(int64_t)j * w + i + 1]);
+        v3 d = xyz(v[(int64_t)(j + 1) * w + i]);
+        v3 n = normalize3(cross3(sub3(r, c), sub3(d, c)));
+        return make_float4(n.x, n.y, n.z, kValid);
+    }
+    return make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+}
+
+// ---------------------------------------------------------------------------
+// TSDF volume layout in HBM: 8x8x8 bricks of 512 voxels, bricks ordered x then
+// y then z, voxels inside a brick ordered x then y then z.  A brick is 2 KiB of
+// FP32 (one plane), so a wave's 64 lanes touch whole 128-byte lines when they
+// sweep a brick, and a trilinear stencil usually stays inside one brick.
+constexpr int kBrick = 8;
+constexpr int kBrickVoxels = 512;
+
+__device__ __forceinline__ int64_t brick_index(int nb, int i, int j, int k)
+{
+    int64_t b = ((int64_t)(k >> 3) * nb + (j >> 3)) * nb + (i >> 3);
+    return b * kBrickVoxels + (((k & 7) * 8 + (j & 7)) * 8 + (i & 7));
+}
+
+// snorm16 TSDF codes (precision form): f = q / 32767 with q in [-32767, 32767].
+constexpr float kSnormScale = 32767.0f;
+__device__ __forceinline__ float snorm_decode(int16_t q) { return (float)q * (1.0f / kSnormScale); }
+__device__ __forceinline__ int16_t snorm_encode(float f)
+{
+    float s = f * kSnormScale;
+    s = fminf(fmaxf(s, -kSnormScale), kSnormScale);
+    return (int16_t)rintf(s);
+}
+
+}  // namespace kf

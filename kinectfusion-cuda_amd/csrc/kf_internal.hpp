@@ -1,0 +1,88 @@
+// Host-side internals of libkfusion: error plumbing, timing helper and the
+// layouts of the opaque handles declared in include/kfusion.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "kfusion.h"
+
+namespace kf {
+
+// Records a detail message for kf_last_error() and returns `status`.
+int fail(int status, const std::string& msg);
+int fail_hip(hipError_t e, const char* what);
+void clear_error();
+
+#define KF_HIP(call)                                             \
+    do {                                                         \
+        hipError_t kf_e_ = (call);                               \
+        if (kf_e_ != hipSuccess) return ::kf::fail_hip(kf_e_, #call); \
+    } while (0)
+
+// Launch-error check right after a kernel launch.
+#define KF_LAUNCHED(name)                                              \
+    do {                                                               \
+        hipError_t kf_e_ = hipGetLastError();                          \
+        if (kf_e_ != hipSuccess) return ::kf::fail_hip(kf_e_, name);   \
+    } while (0)
+
+inline hipStream_t stream_of(kf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// The reference's CudaEvent pair around one wrapper call (cuda_event.cpp:14-30):
+// active only when the caller asked for elapsed milliseconds.
+class CallTimer {
+public:
+    CallTimer(float* elapsed_ms, hipStream_t s);
+    ~CallTimer();
+    int start();
+    int finish();  // records the end event, waits, stores the elapsed time
+
+private:
+    float* out_;
+    hipStream_t s_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+// Device-resident ICP / pose state shared by the ICP loop, the raycast and the
+// fuse of a frame, so a whole frame runs without host round-trips.
+struct alignas(256) PoseState {
+    float R[9];          // current estimate (camera -> world rotation)
+    float t[3];
+    float Rp[9];         // pose at the start of the ICP (previous frame)
+    float tp[3];
+    float pose[16];      // committed sensor pose (camera -> world)
+    float inv_pose[16];  // its inverse (world -> camera)
+    int status;          // first ICP failure of the current frame (kf_status), 0 if none
+    int sticky_status;   // first failure since the host last read it
+    int level_done[4];   // early stop reached at that level
+    int iterations;      // iterations executed in the current frame
+    int frame_skip_fuse; // 1 when the current frame's ICP failed
+    double last_sums[27];
+    long long last_inliers;
+    float last_x[6];
+};
+
+}  // namespace kf
+
+struct kf_volume {
+    int n = 0, nb = 0, format = KF_VOXEL_F32;
+    float width = 0, res = 0, mue = 0;
+    void* f = nullptr;        // float* or int16_t* plane
+    void* w = nullptr;        // float* or uint8_t* plane
+    float* zk = nullptr;      // z coordinate of each k slice (repeated addition, tsdf.cu:27-31)
+    unsigned long long* touched = nullptr;  // device counter of fused voxels
+    size_t bytes = 0;
+};
+
+struct kf_icp {
+    kf_icp_config cfg{};
+    float cos_thresh = 1.0f;
+    kf::PoseState* state = nullptr;  // device
+    float* partials = nullptr;       // device, [max_blocks][kPartialStride]
+    int max_blocks = 0;
+    float times[2] = {0, 0};
+    int last_iterations = 0;
+};

@@ -1,0 +1,46 @@
+// Launch interfaces shared between the library's translation units.  Each
+// launcher enqueues kernels on `s` and returns a kf_status.
+#pragma once
+
+#include "kf_internal.hpp"
+
+namespace kf {
+
+// ---- measurement (measurement.hip)
+int launch_convert(const uint16_t* raw, float* out, int w, int h, float scale, int unmirror, hipStream_t s);
+int launch_bilateral(const float* in, float* out, int w, int h, hipStream_t s);
+int launch_down_sample(const float* in, int w_in, int h_in, float* out, hipStream_t s);
+int launch_vertex_map(const float* depth, float4* out, int w, int h, const float* inv_k, hipStream_t s);
+int launch_normal_map(const float4* v, float4* out, int w, int h, hipStream_t s);
+int launch_preprocess(const uint16_t* raw, int w, int h, float scale, int unmirror, int levels,
+                      const float (*inv_k)[9], float* depth_m, float* const* dpyr, float4* const* vpyr,
+                      float4* const* npyr, hipStream_t s);
+
+// ---- TSDF (tsdf.hip)
+// inv_pose: by-value world->camera matrix; dyn_state (may be null) overrides it
+// with PoseState::inv_pose and skips the launch's work when the frame failed.
+int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
+                const PoseState* dyn_state, hipStream_t s);
+int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s);
+
+// ---- raycast (raycast.hip)
+int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
+                   float4* vout, float4* nout, int w, int h, hipStream_t s);
+
+// ---- ICP (icp.hip)
+constexpr int kPartialStride = 64;  // floats per block partial: 27 sums, 27 compensations, count
+struct IcpLevel {
+    const float4* V;
+    const float4* TV;
+    const float4* TN;
+    int w, h;
+    float K[9];
+};
+int icp_blocks_for(int w, int h);
+// Enqueues the whole coarse-to-fine loop on `s` with the state on the device.
+int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s);
+// begin: copies state->pose into the ICP working pose (or from pose16 if given)
+int launch_icp_begin(PoseState* st, const float* pose16, hipStream_t s);
+int launch_icp_commit(PoseState* st, hipStream_t s);
+
+}  // namespace kf

@@ -1,0 +1,210 @@
+// Raycaster (project/src/raycast.cu): per pixel of a pyramid level, march the
+// TSDF from the sensor pose, find the zero crossing and write the world-space
+// vertex and the normal (-grad F) of the surface.
+//
+// The reference samples the volume through a CUDA texture (normalized
+// coordinates, linear filter, clamp; voxel_grid.cpp:24-33).  Here the same
+// texel-centre rule is evaluated in software on the bricked f plane:
+//   x = u*n - 1/2, i = floor(x), a = x - i, indices clamped to [0, n-1],
+// with exact FP32 weights (the texture unit keeps 8 fractional bits).
+//
+// Launch: one lane per pixel, 8x8 pixel tiles per 64-lane wave (rays of a wave
+// are coherent, so their trilinear stencils share bricks and cache lines);
+// tiles are assigned so that neighbouring tiles run on the same XCD (the 8
+// XCDs each have their own L2 and receive workgroups round-robin).
+#include <string.h>
+
+#include "kf_device.hpp"
+#include "kf_kernels.hpp"
+
+namespace kf {
+
+namespace {
+
+constexpr int kMaxMarchSteps = 1 << 20;
+
+struct RaycastArgs {
+    const void* f;
+    int n, nb, format;
+    float width, mue;
+    float pose[16];
+    const PoseState* dyn;  // when set, the pose comes from dyn->pose
+    float Kinv[9];
+    float4* vout;
+    float4* nout;
+    int w, h;
+    int tiles_x, tiles_y;
+};
+
+template <int FMT>
+__device__ __forceinline__ float voxel_f(const void* f, int64_t q)
+{
+    if (FMT == KF_VOXEL_F32) return static_cast<const float*>(f)[q];
+    return snorm_decode(static_cast<const int16_t*>(f)[q]);
+}
+
+// Trilinear sample at normalized coordinates (u, v, s).
+template <int FMT>
+__device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float u, float v, float s)
+{
+    float x = u * (float)n - 0.5f, y = v * (float)n - 0.5f, z = s * (float)n - 0.5f;
+    float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    float a = x - fx, b = y - fy, c = z - fz;
+    const float lo = -1.0f, hi = (float)n;
+    fx = fx < lo ? lo : (fx > hi ? hi : fx);
+    fy = fy < lo ? lo : (fy > hi ? hi : fy);
+    fz = fz < lo ? lo : (fz > hi ? hi : fz);
+    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+    int i1 = clampi(i0 + 1, 0, n - 1), j1 = clampi(j0 + 1, 0, n - 1), k1 = clampi(k0 + 1, 0, n - 1);
+    i0 = clampi(i0, 0, n - 1);
+    j0 = clampi(j0, 0, n - 1);
+    k0 = clampi(k0, 0, n - 1);
+    float c000 = voxel_f<FMT>(F, brick_index(nb, i0, j0, k0)), c100 = voxel_f<FMT>(F, brick_index(nb, i1, j0, k0));
+    float c010 = voxel_f<FMT>(F, brick_index(nb, i0, j1, k0)), c110 = voxel_f<FMT>(F, brick_index(nb, i1, j1, k0));
+    float c001 = voxel_f<FMT>(F, brick_index(nb, i0, j0, k1)), c101 = voxel_f<FMT>(F, brick_index(nb, i1, j0, k1));
+    float c011 = voxel_f<FMT>(F, brick_index(nb, i0, j1, k1)), c111 = voxel_f<FMT>(F, brick_index(nb, i1, j1, k1));
+    float e00 = (1.0f - a) * c000 + a * c100, e10 = (1.0f - a) * c010 + a * c110;
+    float e01 = (1.0f - a) * c001 + a * c101, e11 = (1.0f - a) * c011 + a * c111;
+    float g0 = (1.0f - b) * e00 + b * e10, g1 = (1.0f - b) * e01 + b * e11;
+    return (1.0f - c) * g0 + c * g1;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
+{
+    // XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
+    // contiguous band of tiles.
+    const int tiles = a.tiles_x * a.tiles_y;
+    const int g = blockIdx.x;
+    const int per_xcd = (tiles + 7) / 8;
+    const int tile = (g % 8) * per_xcd + g / 8;
+    if (tile >= tiles) return;
+    const int i = (tile % a.tiles_x) * 8 + (threadIdx.x & 7);
+    const int j = (tile / a.tiles_x) * 8 + (threadIdx.x >> 3);
+    if (i >= a.w || j >= a.h) return;
+
+    const float* pose = a.dyn ? a.dyn->pose : a.pose;
+    float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
+    float M[9];
+    m3m3(Rm, a.Kinv, M);
+    const v3 o = mk3(pose[12], pose[13], pose[14]);
+    const float width = a.width, mue = a.mue, hw = width * 0.5f;
+
+    v3 dir = m3v(M, mk3((float)i + 0.5f, (float)j + 0.5f, 1.0f));
+    v3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    // raycast.cu:7-27
+    float t0x = (-hw - o.x) * inv.x, t1x = (hw - o.x) * inv.x;
+    float t0y = (-hw - o.y) * inv.y, t1y = (hw - o.y) * inv.y;
+    float t0z = (-hw - o.z) * inv.z, t1z = (hw - o.z) * inv.z;
+    float tmin = fminf(t0x, t1x), tmax = fmaxf(t0x, t1x);
+    tmin = fmaxf(tmin, fminf(t0y, t1y));
+    tmax = fminf(tmax, fmaxf(t0y, t1y));
+    tmin = fmaxf(tmin, fminf(t0z, t1z));
+    tmax = fminf(tmax, fmaxf(t0z, t1z));
+    if (tmax < tmin) {
+        tmin = -1.0f;
+        tmax = -1.0f;
+    }
+    // raycast.cu:61-73: leave the pixel untouched
+    if (tmax < 0.0f) return;
+    const float nearv = (kMinDistance < tmin) ? tmin : kMinDistance;
+    const float farv = (tmax < kMaxDistance) ? tmax : kMaxDistance;
+    if (nearv >= farv) return;
+
+    // raycast.cu:75-117
+    const float large = mue * 1.99f, small = mue * 0.1f;
+    float step = large, prev = 0.0f, tstar = 0.0f;
+    int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
+    for (float t = nearv; t < farv; t = t + step) {
+        if (++guard > kMaxMarchSteps) break;
+        v3 p = add3(o, mul3s(dir, t));
+        float f = sample_tsdf<FMT>(a.f, a.n, a.nb, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+        if (f < 0.0f) {
+            if (t != nearv) {
+                if (step == large) {
+                    t = t - mue;
+                    step = small;
+                    continue;
+                }
+                tstar = t - step * prev / (f - prev);
+            }
+            break;
+        } else if (f < 0.99f) {
+            step = small;
+        }
+        prev = f;
+    }
+
+    const int64_t q = (int64_t)j * a.w + i;
+    if (tstar != 0.0f) {
+        v3 vtx = add3(o, mul3s(dir, tstar));
+        // raycast.cu:29-42
+        float r = mue * 0.5f / width;
+        float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
+        float fx0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u + r, v, s);
+        float fy0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v + r, s);
+        float fz0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s + r);
+        float fx = sample_tsdf<FMT>(a.f, a.n, a.nb, u - r, v, s);
+        float fy = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v - r, s);
+        float fz = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s - r);
+        v3 g = div3s(mk3(fx - fx0, fy - fy0, fz - fz0), r);
+        float gn = len3(g);
+        if (is_depth_valid(gn)) {
+            v3 nn = div3s(g, gn);
+            a.vout[q] = make_float4(vtx.x, vtx.y, vtx.z, kValid);
+            a.nout[q] = make_float4(nn.x, nn.y, nn.z, kValid);
+            return;
+        }
+    }
+    a.vout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+    a.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+}
+
+}  // namespace
+
+int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
+                   float4* vout, float4* nout, int w, int h, hipStream_t s)
+{
+    RaycastArgs a{};
+    a.f = v->f;
+    a.n = v->n;
+    a.nb = v->nb;
+    a.format = v->format;
+    a.width = v->width;
+    a.mue = v->mue;
+    if (pose) memcpy(a.pose, pose, sizeof(a.pose));
+    a.dyn = dyn_state;
+    memcpy(a.Kinv, inv_k, sizeof(a.Kinv));
+    a.vout = vout;
+    a.nout = nout;
+    a.w = w;
+    a.h = h;
+    a.tiles_x = (w + 7) / 8;
+    a.tiles_y = (h + 7) / 8;
+    const int tiles = a.tiles_x * a.tiles_y;
+    const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
+    if (v->format == KF_VOXEL_F32)
+        raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
+    else
+        raycast_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(a);
+    KF_LAUNCHED("raycast_kernel");
+    return KF_OK;
+}
+
+}  // namespace kf
+
+extern "C" int kf_raycast(const kf_volume* v, const kf_sensor* sensor, float* vertex_map, float* normal_map, int w,
+                          int h, int level, kf_stream_t stream, float* elapsed_ms)
+{
+    if (!v || !sensor || !vertex_map || !normal_map || w <= 0 || h <= 0)
+        return kf::fail(KF_ERR_ARG, "raycast: bad argument");
+    if (level < 0 || level > 2) return kf::fail(KF_ERR_ARG, "raycast: level must be 0..2 (sensor.h:51-52)");
+    hipStream_t s = kf::stream_of(stream);
+    kf::CallTimer t(elapsed_ms, s);
+    int st;
+    if ((st = t.start()) ||
+        (st = kf::launch_raycast(v, sensor->pose, nullptr, sensor->inv_intr[level], reinterpret_cast<float4*>(vertex_map),
+                                 reinterpret_cast<float4*>(normal_map), w, h, s)))
+        return st;
+    return t.finish();
+}
