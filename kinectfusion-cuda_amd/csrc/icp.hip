@@ -12,6 +12,8 @@
 //              result is deterministic), solves the 6x6 system with an FP64
 //              Cholesky, applies the pose update (icp.cpp:65-87) and the early
 //              stop test (icp.cpp:45-53) to the device-resident PoseState.
+// The kernel boundary between the two launches is what makes every
+// workgroup's partial row visible to the finalize across the 8 XCDs' L2s.
 // The reference instead writes A (N x 6) and b to HBM, runs cublasSgemm and
 // cublasSgemv, copies 42 floats to the host and solves with Eigen's FP32 LLT,
 // with a host synchronisation per iteration.
@@ -46,26 +48,45 @@ struct IcpArgs {
     int stride;     // blocks of this launch (partial row stride)
 };
 
-// icp.cu:7-82 for pixel (u, v); false = zero row.
-__device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const float* Rc, const float* tc,
-                                        const float* M, const float* tp, float* row, float& b)
+// icp.cu:7-82 split in two: what a pixel needs that does not depend on the
+// pose (its vertex and its live normal, device_helper.cuh:81-110) and the
+// pose-dependent association, so the loads of the first part can be issued
+// before the pose of the iteration is known.
+struct PixelIn {
+    v3 vc;       // camera-space vertex
+    v3 nl;       // live normal
+    bool ok;     // vertex and normal valid
+};
+__device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
 {
-    if (!map_valid(a.V, a.w, a.h, u, v)) return false;
-    v3 vc = xyz(a.V[(int64_t)v * a.w + u]);
-    v3 s = add3(m3v(Rc, vc), mk3(tc[0], tc[1], tc[2]));
+    PixelIn in;
+    in.ok = map_valid(a.V, a.w, a.h, u, v);
+    in.vc = in.ok ? xyz(a.V[(int64_t)v * a.w + u]) : mk3(0.0f, 0.0f, 0.0f);
+    float4 nl = compute_normal(a.V, a.w, a.h, u, v);
+    in.nl = xyz(nl);
+    in.ok = in.ok && nl.w != kInvalid;
+    return in;
+}
+
+// false = zero row (rejection reasons as icp.cu:29-70, evaluated in an order
+// that does not change the result: every rejection writes the same zero row).
+__device__ __forceinline__ bool icp_pixel_row(const IcpArgs& a, const PixelIn& in, const float* Rc, const float* tc,
+                                              const float* M, const float* tp, float* row, float& b)
+{
+    if (!in.ok) return false;
+    v3 s = add3(m3v(Rc, in.vc), mk3(tc[0], tc[1], tc[2]));
     // icp.cuh:40-49: (K * R_prev^T) * (s - t_prev), floor of the perspective division
     v3 p = m3v(M, sub3(s, mk3(tp[0], tp[1], tp[2])));
     float fu = floorf(p.x / p.z), fv = floorf(p.y / p.z);
     if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)a.w && fv < (float)a.h)) return false;
     int uc = (int)fu, vcr = (int)fv;
-    if (!map_valid(a.TV, a.w, a.h, uc, vcr) || !map_valid(a.TN, a.w, a.h, uc, vcr)) return false;
-    v3 d = xyz(a.TV[(int64_t)vcr * a.w + uc]);
+    const float4 tv = a.TV[(int64_t)vcr * a.w + uc], tn = a.TN[(int64_t)vcr * a.w + uc];
+    if (tv.w != kValid || tn.w != kValid) return false;
+    v3 d = xyz(tv);
     if (len3(sub3(d, s)) > a.dist_thresh) return false;  // icp.cuh:57-60
-    v3 n = xyz(a.TN[(int64_t)vcr * a.w + uc]);
-    float4 nl = compute_normal(a.V, a.w, a.h, u, v);
-    if (nl.w == kInvalid) return false;
+    v3 n = xyz(tn);
     // icp.cuh:63-70 in its cosine form: acos(d) > th  <=>  d >= -1 && d < cos(th)
-    v3 nn = normalize3(m3v(Rc, xyz(nl)));
+    v3 nn = normalize3(m3v(Rc, in.nl));
     float dd = dot3(nn, n);
     if (dd >= -1.0f && dd < a.cos_thresh) return false;
     // icp.cuh:72-95
@@ -77,6 +98,12 @@ __device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const fl
     row[5] = n.z;
     b = ((((n.x * d.x + n.y * d.y) + n.z * d.z) - n.x * s.x) - n.y * s.y) - n.z * s.z;
     return true;
+}
+
+__device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const float* Rc, const float* tc,
+                                        const float* M, const float* tp, float* row, float& b)
+{
+    return icp_pixel_row(a, icp_pixel_in(a, u, v), Rc, tc, M, tp, row, b);
 }
 
 // s += v with the rounding error of the addition carried in c (TwoSum).
@@ -161,41 +188,69 @@ __device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, 
     }
 }
 
-template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_reduce_kernel(IcpArgs a)
+// The pixels of lane (block, k, lane) and their pose-independent inputs.
+struct LanePixels {
+    PixelIn in[kPixPerThread];
+    bool live[kPixPerThread];
+};
+__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp)
 {
-    PoseState* st = a.st;
-    if (st->status != 0 || st->level_done[a.level]) return;
-    __shared__ float pose[24 + 9];
-    if (threadIdx.x < 9) pose[threadIdx.x] = st->R[threadIdx.x];
-    else if (threadIdx.x < 12) pose[threadIdx.x] = st->t[threadIdx.x - 9];
-    else if (threadIdx.x < 21) pose[threadIdx.x] = st->Rp[threadIdx.x - 12];
-    else if (threadIdx.x < 24) pose[threadIdx.x] = st->tp[threadIdx.x - 21];
-    __syncthreads();
+    const int64_t npx = (int64_t)a.w * a.h;
+#pragma unroll
+    for (int k = 0; k < kPixPerThread; ++k) {
+        int64_t px = ((int64_t)blockIdx.x * kPixPerThread + k) * kThreads + threadIdx.x;
+        lp.live[k] = px < npx;
+        int u = lp.live[k] ? (int)(px % a.w) : 0, v = lp.live[k] ? (int)(px / a.w) : 0;
+        lp.in[k] = icp_pixel_in(a, u, v);
+        lp.in[k].ok = lp.in[k].ok && lp.live[k];
+    }
+}
+
+// pose: R (9), t (3), Rp (9), tp (3) of the iteration, in LDS
+template <bool COMP>
+__device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const float* pose)
+{
     const float* Rc = pose;
     const float* tc = pose + 9;
     float Rt[9], M[9];
     m3t(pose + 12, Rt);
     m3m3(a.K, Rt, M);
     const float* tp = pose + 21;
-
     float s[kTerms], c[kTerms];
 #pragma unroll
     for (int q = 0; q < kTerms; ++q) s[q] = c[q] = 0.0f;
     int cnt = 0;
-    const int64_t npx = (int64_t)a.w * a.h;
 #pragma unroll
     for (int k = 0; k < kPixPerThread; ++k) {
-        int64_t px = ((int64_t)blockIdx.x * kPixPerThread + k) * kThreads + threadIdx.x;
-        if (px >= npx) break;
-        int u = (int)(px % a.w), v = (int)(px / a.w);
         float row[6], b;
-        if (icp_row(a, u, v, Rc, tc, M, tp, row, b)) {
+        if (icp_pixel_row(a, lp.in[k], Rc, tc, M, tp, row, b)) {
             accumulate<COMP>(s, c, row, b);
             ++cnt;
         }
     }
     block_reduce_store<COMP>(s, c, cnt, a.partials, a.stride);
+}
+
+template <bool COMP>
+__device__ void reduce_body(const IcpArgs& a)
+{
+    PoseState* st = a.st;
+    __shared__ float pose[24];
+    LanePixels lp;
+    load_lane_pixels(a, lp);
+    if (threadIdx.x < 9) pose[threadIdx.x] = st->R[threadIdx.x];
+    else if (threadIdx.x < 12) pose[threadIdx.x] = st->t[threadIdx.x - 9];
+    else if (threadIdx.x < 21) pose[threadIdx.x] = st->Rp[threadIdx.x - 12];
+    else if (threadIdx.x < 24) pose[threadIdx.x] = st->tp[threadIdx.x - 21];
+    __syncthreads();
+    associate_reduce<COMP>(a, lp, pose);
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_reduce_kernel(IcpArgs a)
+{
+    if (a.st->status != 0 || a.st->level_done[a.level]) return;
+    reduce_body<COMP>(a);
 }
 
 // Rows of a given A (n x 6 row-major) and b (LinearLeastSquares::solve).
@@ -356,13 +411,36 @@ __device__ void expand(const double* tot, double* ata, double* atb)
     for (int p = 0; p < 6; ++p) atb[p] = tot[21 + p];
 }
 
-// mode 0: ICP iteration (solve + pose update + early stop); mode 1: solve only.
-template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, const float* partials, int stride,
-                                                                int nblocks, int level, float stop_angle,
-                                                                float stop_dist, int mode)
+// Solve of one iteration from its 27 sums (thread 0): FP64 Cholesky, pose
+// update (icp.cpp:65-87) and the early-stop test (icp.cpp:45-53).  Returns 0 or
+// a kf_status; x receives the float update.
+__device__ int solve_update(const double* tot, float* R, float* t, int* level_done, float stop_angle,
+                            float stop_dist, float* x)
 {
-    if (mode == 0 && (st->status != 0 || st->level_done[level])) return;
+    double ata[36], atb[6], xd[6];
+    expand(tot, ata, atb);
+    if (chol6(ata, atb, xd) != 0) return KF_ERR_NOT_PD;
+    bool finite = true;
+    for (int q = 0; q < 6; ++q) {
+        x[q] = (float)xd[q];
+        finite = finite && isfinite(x[q]);
+    }
+    if (!finite) return KF_ERR_NONFINITE;
+    float Rprev[9], tprev[3], err[2];
+    for (int q = 0; q < 9; ++q) Rprev[q] = R[q];
+    for (int q = 0; q < 3; ++q) tprev[q] = t[q];
+    update_pose(R, t, x);
+    pose_error(Rprev, tprev, R, t, err);
+    if (err[0] < stop_angle && err[1] < stop_dist) *level_done = 1;
+    return KF_OK;
+}
+
+// mode 0: ICP iteration (solve + pose update + early stop); mode 1: solve only.
+// Executed by every lane of one workgroup.
+template <bool COMP>
+__device__ void finalize_body(PoseState* st, const float* partials, int stride, int nblocks, int level,
+                              float stop_angle, float stop_dist, int mode)
+{
     __shared__ double tot[kTerms];
     __shared__ double count;
     sum_partials<COMP>(partials, stride, nblocks, tot, &count);
@@ -400,44 +478,150 @@ __global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, c
     if (err[0] < stop_angle && err[1] < stop_dist) st->level_done[level] = 1;
 }
 
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, const float* partials, int stride,
+                                                                int nblocks, int level, float stop_angle,
+                                                                float stop_dist, int mode)
+{
+    if (mode == 0 && (st->status != 0 || st->level_done[level])) return;
+    finalize_body<COMP>(st, partials, stride, nblocks, level, stop_angle, stop_dist, mode);
+}
+
+// ---- the device-resident coarse-to-fine loop: one launch per iteration.
+// Launch k first finishes iteration k-1 (every workgroup sums launch k-1's
+// partial rows in the same fixed order and solves, so all of them hold the same
+// pose), then associates and reduces iteration k.  The carried state and the
+// partial rows are double-buffered by launch parity, so no workgroup reads what
+// another workgroup of the same launch writes.
+}  // namespace
+struct IcpCarry {
+    float R[9];
+    float t[3];
+    int status;
+    int level_done[4];
+    int iterations;
+    int pending;         // the writing launch left partial rows that still need their solve
+    int pending_level;
+    int pending_blocks;  // partial-row count of that launch
+    int pad[2];
+};
+namespace {
+
+// Applies the pending solve of `c` (thread 0; tot/count already summed).
+__device__ void apply_pending(IcpCarry& c, const double* tot, double count, PoseState* st, bool record,
+                              float stop_angle, float stop_dist)
+{
+    if (!c.pending || c.status != 0) return;
+    float x[6] = {0, 0, 0, 0, 0, 0};
+    int rc = solve_update(tot, c.R, c.t, &c.level_done[c.pending_level], stop_angle, stop_dist, x);
+    if (rc) c.status = rc;
+    else c.iterations += 1;
+    if (record) {
+        for (int q = 0; q < kTerms; ++q) st->last_sums[q] = tot[q];
+        st->last_inliers = (long long)count;
+        for (int q = 0; q < 6; ++q) st->last_x[q] = x[q];
+    }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const IcpCarry* cin, IcpCarry* cout,
+                                                            const float* pin, float stop_angle, float stop_dist)
+{
+    __shared__ float pose[24];
+    __shared__ int run;
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    PoseState* st = a.st;
+    // phase A: pose-independent loads of this lane's pixels
+    LanePixels lp;
+    load_lane_pixels(a, lp);
+    // phase B: finish the previous iteration
+    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, &count);
+    if (threadIdx.x == 0) {
+        IcpCarry c = *cin;
+        apply_pending(c, tot, count, st, blockIdx.x == 0, stop_angle, stop_dist);
+        const int go = (c.status == 0 && !c.level_done[a.level]) ? 1 : 0;
+        c.pending = go;
+        c.pending_level = a.level;
+        c.pending_blocks = (int)gridDim.x;
+        for (int q = 0; q < 9; ++q) pose[q] = c.R[q];
+        for (int q = 0; q < 3; ++q) pose[9 + q] = c.t[q];
+        for (int q = 0; q < 9; ++q) pose[12 + q] = st->Rp[q];
+        for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
+        run = go;
+        if (blockIdx.x == 0) *cout = c;
+    }
+    __syncthreads();
+    if (!run) return;
+    // phase C: association + reduction of this iteration
+    associate_reduce<COMP>(a, lp, pose);
+}
+
 // Starts a frame's ICP: the working and previous poses are the sensor pose
 // (icp.cpp:27-29); from `pose` when given by value, else from st->pose.
 struct Pose16 {
     float m[16];
 };
-__global__ void icp_begin_kernel(PoseState* st, Pose16 pose, int from_value)
+__global__ void icp_begin_kernel(PoseState* st, IcpCarry* c0, Pose16 pose, int from_value)
 {
     if (threadIdx.x != 0) return;
     if (from_value)
         for (int q = 0; q < 16; ++q) st->pose[q] = pose.m[q];
     const float* P = st->pose;
     float Rm[9] = {P[0], P[1], P[2], P[4], P[5], P[6], P[8], P[9], P[10]};
-    for (int q = 0; q < 9; ++q) st->R[q] = st->Rp[q] = Rm[q];
-    for (int q = 0; q < 3; ++q) st->t[q] = st->tp[q] = P[12 + q];
+    IcpCarry c;
+    for (int q = 0; q < 9; ++q) st->R[q] = st->Rp[q] = c.R[q] = Rm[q];
+    for (int q = 0; q < 3; ++q) st->t[q] = st->tp[q] = c.t[q] = P[12 + q];
     st->status = 0;
-    for (int l = 0; l < 4; ++l) st->level_done[l] = 0;
+    for (int l = 0; l < 4; ++l) st->level_done[l] = c.level_done[l] = 0;
     st->iterations = 0;
     st->frame_skip_fuse = 0;
+    c.status = 0;
+    c.iterations = 0;
+    c.pending = 0;
+    c.pending_level = 0;
+    c.pending_blocks = 0;
+    c.pad[0] = c.pad[1] = 0;
+    *c0 = c;
 }
 
-// Sensor::setPose of the estimate (kinect_fusion.cpp:146-161, sensor.h:30-34).
-__global__ void icp_commit_kernel(PoseState* st)
+// Finishes the last iteration and commits the estimate as the sensor pose
+// (Sensor::setPose, kinect_fusion.cpp:146-161, sensor.h:30-34); on failure the
+// previous pose is kept and the frame is not fused.
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, const IcpCarry* cin, const float* pin,
+                                                              float stop_angle, float stop_dist)
 {
-    if (threadIdx.x != 0 || st->status != 0) return;
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, &count);
+    if (threadIdx.x != 0) return;
+    IcpCarry c = *cin;
+    apply_pending(c, tot, count, st, true, stop_angle, stop_dist);
+    for (int q = 0; q < 9; ++q) st->R[q] = c.R[q];
+    for (int q = 0; q < 3; ++q) st->t[q] = c.t[q];
+    for (int l = 0; l < 4; ++l) st->level_done[l] = c.level_done[l];
+    st->iterations = c.iterations;
+    st->status = c.status;
+    if (c.status != 0) {
+        if (st->sticky_status == 0) st->sticky_status = c.status;
+        st->frame_skip_fuse = 1;
+        return;
+    }
     float* P = st->pose;
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) P[c * 4 + r] = st->R[c * 3 + r];
+    for (int cc = 0; cc < 3; ++cc)
+        for (int r = 0; r < 3; ++r) P[cc * 4 + r] = c.R[cc * 3 + r];
     P[3] = P[7] = P[11] = 0.0f;
-    for (int r = 0; r < 3; ++r) P[12 + r] = st->t[r];
+    for (int r = 0; r < 3; ++r) P[12 + r] = c.t[r];
     P[15] = 1.0f;
     float* I = st->inv_pose;
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) I[c * 4 + r] = st->R[r * 3 + c];
+    for (int cc = 0; cc < 3; ++cc)
+        for (int r = 0; r < 3; ++r) I[cc * 4 + r] = c.R[r * 3 + cc];
     I[3] = I[7] = I[11] = 0.0f;
     for (int r = 0; r < 3; ++r) {
-        double s = 0;
-        for (int q = 0; q < 3; ++q) s += (double)st->R[r * 3 + q] * (double)st->t[q];
-        I[12 + r] = (float)(-s);
+        double sum = 0;
+        for (int q = 0; q < 3; ++q) sum += (double)c.R[r * 3 + q] * (double)c.t[q];
+        I[12 + r] = (float)(-sum);
     }
     I[15] = 1.0f;
 }
@@ -448,60 +632,67 @@ inline int blocks_for(int64_t n) { return (int)((n + kThreads * kPixPerThread - 
 
 int icp_blocks_for(int w, int h) { return blocks_for((int64_t)w * h); }
 
-int launch_icp_begin(PoseState* st, const float* pose16, hipStream_t s)
+static IcpCarry* carry_of(kf_icp* icp, int k) { return reinterpret_cast<IcpCarry*>(icp->carry) + (k & 1); }
+static float* partials_of(kf_icp* icp, int k)
+{
+    return icp->partials + (size_t)(k & 1) * kPartialStride * (size_t)icp->max_blocks;
+}
+
+int launch_icp_begin(kf_icp* icp, const float* pose16, hipStream_t s)
 {
     Pose16 p{};
     if (pose16) memcpy(p.m, pose16, sizeof(p.m));
-    icp_begin_kernel<<<1, 64, 0, s>>>(st, p, pose16 ? 1 : 0);
+    icp_begin_kernel<<<1, 64, 0, s>>>(icp->state, carry_of(icp, 0), p, pose16 ? 1 : 0);
     KF_LAUNCHED("icp_begin_kernel");
     return KF_OK;
 }
 
-int launch_icp_commit(PoseState* st, hipStream_t s)
+// Enqueues the iteration launches; `events` (optional) receives one event
+// after every launch.  Returns the number of launches in *launches.
+static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s, hipEvent_t* events,
+                         int* launches)
 {
-    icp_commit_kernel<<<1, 64, 0, s>>>(st);
-    KF_LAUNCHED("icp_commit_kernel");
-    return KF_OK;
-}
-
-static int launch_iteration(kf_icp* icp, const IcpLevel& L, int level, hipStream_t s, hipEvent_t mid)
-{
-    IcpArgs a{};
-    a.V = L.V;
-    a.TV = L.TV;
-    a.TN = L.TN;
-    a.w = L.w;
-    a.h = L.h;
-    memcpy(a.K, L.K, sizeof(a.K));
-    a.dist_thresh = icp->cfg.distance_thresh;
-    a.cos_thresh = icp->cos_thresh;
-    a.st = icp->state;
-    a.level = level;
-    a.partials = icp->partials;
-    const int blocks = blocks_for((int64_t)L.w * L.h);
-    a.stride = blocks;
-    if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
-    if (icp->cfg.compensated) icp_reduce_kernel<true><<<blocks, kThreads, 0, s>>>(a);
-    else icp_reduce_kernel<false><<<blocks, kThreads, 0, s>>>(a);
-    KF_LAUNCHED("icp_reduce_kernel");
-    if (mid) KF_HIP(hipEventRecord(mid, s));
     const float sa = icp->cfg.iteration_stop_thresh_angle, sd = icp->cfg.iteration_stop_thresh_distance;
+    int k = 0;
+    for (int l = n_levels - 1; l >= 0; --l)
+        for (int it = 0; it < icp->cfg.iters_per_layer[l]; ++it, ++k) {
+            const IcpLevel& L = levels[l];
+            IcpArgs a{};
+            a.V = L.V;
+            a.TV = L.TV;
+            a.TN = L.TN;
+            a.w = L.w;
+            a.h = L.h;
+            memcpy(a.K, L.K, sizeof(a.K));
+            a.dist_thresh = icp->cfg.distance_thresh;
+            a.cos_thresh = icp->cos_thresh;
+            a.st = icp->state;
+            a.level = l;
+            a.partials = partials_of(icp, k);
+            const int blocks = blocks_for((int64_t)L.w * L.h);
+            a.stride = blocks;
+            if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
+            if (icp->cfg.compensated)
+                icp_step_kernel<true><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
+                                                                 partials_of(icp, k + 1), sa, sd);
+            else
+                icp_step_kernel<false><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
+                                                                  partials_of(icp, k + 1), sa, sd);
+            KF_LAUNCHED("icp_step_kernel");
+            if (events) KF_HIP(hipEventRecord(events[k], s));
+        }
     if (icp->cfg.compensated)
-        icp_finalize_kernel<true><<<1, kThreads, 0, s>>>(icp->state, icp->partials, blocks, blocks, level, sa, sd, 0);
+        icp_commit_kernel<true><<<1, kThreads, 0, s>>>(icp->state, carry_of(icp, k), partials_of(icp, k + 1), sa, sd);
     else
-        icp_finalize_kernel<false><<<1, kThreads, 0, s>>>(icp->state, icp->partials, blocks, blocks, level, sa, sd, 0);
-    KF_LAUNCHED("icp_finalize_kernel");
+        icp_commit_kernel<false><<<1, kThreads, 0, s>>>(icp->state, carry_of(icp, k), partials_of(icp, k + 1), sa, sd);
+    KF_LAUNCHED("icp_commit_kernel");
+    if (launches) *launches = k;
     return KF_OK;
 }
 
 int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s)
 {
-    for (int l = n_levels - 1; l >= 0; --l)
-        for (int it = 0; it < icp->cfg.iters_per_layer[l]; ++it) {
-            int st = launch_iteration(icp, levels[l], l, s, nullptr);
-            if (st) return st;
-        }
-    return KF_OK;
+    return enqueue_steps(icp, levels, n_levels, s, nullptr, nullptr);
 }
 
 }  // namespace kf
@@ -676,7 +867,9 @@ int kf_icp_create(const kf_icp_config* config, kf_icp** out)
     hipError_t e;
     if ((e = hipMalloc(&icp->state, sizeof(kf::PoseState))) != hipSuccess ||
         (e = hipMemset(icp->state, 0, sizeof(kf::PoseState))) != hipSuccess ||
-        (e = hipMalloc(&icp->partials, sizeof(float) * kf::kPartialStride * (size_t)icp->max_blocks)) != hipSuccess) {
+        (e = hipMalloc(&icp->carry, 2 * sizeof(kf::IcpCarry))) != hipSuccess ||
+        (e = hipMemset(icp->carry, 0, 2 * sizeof(kf::IcpCarry))) != hipSuccess ||
+        (e = hipMalloc(&icp->partials, 2 * sizeof(float) * kf::kPartialStride * (size_t)icp->max_blocks)) != hipSuccess) {
         kf_icp_destroy(icp);
         return kf::fail_hip(e, "kf_icp_create");
     }
@@ -688,6 +881,7 @@ int kf_icp_destroy(kf_icp* icp)
 {
     if (!icp) return KF_OK;
     (void)hipFree(icp->state);
+    (void)hipFree(icp->carry);
     (void)hipFree(icp->partials);
     delete icp;
     return KF_OK;
@@ -709,23 +903,19 @@ int kf_icp_compute_pose(kf_icp* icp, const float* const vertex_pyr[], const floa
         lv[l].h = icp->cfg.height >> l;
         memcpy(lv[l].K, sensor->intr[l], sizeof(lv[l].K));
     }
-    // Events around every association+reduction and every solve (ICP::getExecutionTimes).
+    // Events after every launch (ICP::getExecutionTimes): times[0] = the iteration
+    // launches (association, reduction and the solve of the previous iteration),
+    // times[1] = the final solve and commit.
     int total = 0;
     for (int l = 0; l < icp->cfg.levels; ++l) total += icp->cfg.iters_per_layer[l];
-    std::vector<hipEvent_t> ev((size_t)total * 3 + 2, nullptr);
+    std::vector<hipEvent_t> ev((size_t)total + 2, nullptr);
     int rc = KF_OK;
     for (auto& e : ev)
         if (rc == KF_OK && hipEventCreate(&e) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event create");
-    if (rc == KF_OK) rc = kf::launch_icp_begin(icp->state, sensor->pose, s);
+    if (rc == KF_OK) rc = kf::launch_icp_begin(icp, sensor->pose, s);
     if (rc == KF_OK && hipEventRecord(ev[0], s) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event record");
     int k = 0;
-    for (int l = icp->cfg.levels - 1; l >= 0 && rc == KF_OK; --l)
-        for (int it = 0; it < icp->cfg.iters_per_layer[l] && rc == KF_OK; ++it, ++k) {
-            (void)hipEventRecord(ev[1 + 3 * k], s);
-            rc = kf::launch_iteration(icp, lv[l], l, s, ev[2 + 3 * k]);
-            (void)hipEventRecord(ev[3 + 3 * k], s);
-        }
-    if (rc == KF_OK) rc = kf::launch_icp_commit(icp->state, s);
+    if (rc == KF_OK) rc = kf::enqueue_steps(icp, lv, icp->cfg.levels, s, ev.data() + 1, &k);
     if (rc == KF_OK && hipEventRecord(ev.back(), s) != hipSuccess) rc = kf::fail(KF_ERR_HIP, "ICP event record");
     kf::PoseState hs;
     if (rc == KF_OK) {
@@ -735,13 +925,8 @@ int kf_icp_compute_pose(kf_icp* icp, const float* const vertex_pyr[], const floa
     }
     if (rc == KF_OK) {
         icp->times[0] = icp->times[1] = 0.0f;
-        for (int q = 0; q < k; ++q) {
-            float a = 0, b = 0;
-            (void)hipEventElapsedTime(&a, ev[1 + 3 * q], ev[2 + 3 * q]);
-            (void)hipEventElapsedTime(&b, ev[2 + 3 * q], ev[3 + 3 * q]);
-            icp->times[0] += a;
-            icp->times[1] += b;
-        }
+        if (k > 0) (void)hipEventElapsedTime(&icp->times[0], ev[0], ev[k]);
+        (void)hipEventElapsedTime(&icp->times[1], ev[k], ev.back());
         if (elapsed_ms) (void)hipEventElapsedTime(elapsed_ms, ev[0], ev.back());
         icp->last_iterations = hs.iterations;
         // the last good estimate (the commit does not run on failure)

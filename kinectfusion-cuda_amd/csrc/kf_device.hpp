@@ -152,6 +152,16 @@ __device__ __forceinline__ int64_t brick_index(int nb, int i, int j, int k)
     return b * kBrickVoxels + (((k & 7) * 8 + (j & 7)) * 8 + (i & 7));
 }
 
+// Super-bricks: 8x8x8 bricks (64^3 voxels) carry a lower bound of F too, kept
+// as an ordered integer key so it can be lowered with an integer atomicMin.
+constexpr int kSuper = 64;
+__device__ __forceinline__ int order_key(float f)
+{
+    int b = __float_as_int(f);
+    return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+__device__ __forceinline__ float order_value(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
 // snorm16 TSDF codes (precision form): f = q / 32767 with q in [-32767, 32767].
 constexpr float kSnormScale = 32767.0f;
 __device__ __forceinline__ float snorm_decode(int16_t q) { return (float)q * (1.0f / kSnormScale); }

@@ -73,7 +73,13 @@ struct kf_volume {
     void* f = nullptr;        // float* or int16_t* plane
     void* w = nullptr;        // float* or uint8_t* plane
     float* zk = nullptr;      // z coordinate of each k slice (repeated addition, tsdf.cu:27-31)
-    unsigned long long* touched = nullptr;  // device counter of fused voxels
+    unsigned long long* touched = nullptr;  // per-workgroup counts of fused voxels (summed on read)
+    float* brick_min = nullptr;             // per-brick lower bound of F (raycast skipping)
+    int* super_key = nullptr;               // per-super-brick lower bound of F (order_key)
+    int ns = 0;                             // super-bricks per axis (ceil(nb / 8))
+    int* brick_list = nullptr;              // visible bricks of the current fuse
+    unsigned int* list_count = nullptr;     // list lengths, double-buffered by launch parity
+    int64_t fuse_launches = 0;
     size_t bytes = 0;
 };
 
@@ -81,7 +87,8 @@ struct kf_icp {
     kf_icp_config cfg{};
     float cos_thresh = 1.0f;
     kf::PoseState* state = nullptr;  // device
-    float* partials = nullptr;       // device, [max_blocks][kPartialStride]
+    void* carry = nullptr;           // device, 2 x IcpCarry (double-buffered by launch parity)
+    float* partials = nullptr;       // device, 2 x [kPartialStride][max_blocks]
     int max_blocks = 0;
     float times[2] = {0, 0};
     int last_iterations = 0;

@@ -26,6 +26,10 @@ int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s);
 // ---- raycast (raycast.hip)
 int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
                    float4* vout, float4* nout, int w, int h, hipStream_t s);
+// All pyramid levels in one launch (level l is w0 >> l by h0 >> l).
+int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState* dyn_state, int levels,
+                          const float (*inv_k)[9], float4* const* vout, float4* const* nout, int w0, int h0,
+                          hipStream_t s);
 
 // ---- ICP (icp.hip)
 constexpr int kPartialStride = 64;  // floats per block partial: 27 sums, 27 compensations, count
@@ -37,10 +41,11 @@ struct IcpLevel {
     float K[9];
 };
 int icp_blocks_for(int w, int h);
-// Enqueues the whole coarse-to-fine loop on `s` with the state on the device.
+// Enqueues the whole coarse-to-fine loop on `s` with the state on the device,
+// ending with the commit of the estimate into state->pose / inv_pose.
 int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s);
-// begin: copies state->pose into the ICP working pose (or from pose16 if given)
-int launch_icp_begin(PoseState* st, const float* pose16, hipStream_t s);
-int launch_icp_commit(PoseState* st, hipStream_t s);
+// begin: starts the frame's ICP from state->pose (or from pose16 if given);
+// launch_icp_loop then enqueues one launch per iteration and the final commit.
+int launch_icp_begin(kf_icp* icp, const float* pose16, hipStream_t s);
 
 }  // namespace kf

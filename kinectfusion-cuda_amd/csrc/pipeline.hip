@@ -172,9 +172,8 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
             return st;
         if (p->timing) KF_HIP(hipEventRecord(ev[1], s));
         // raycastTsdf (kinect_fusion.cpp:137-144) at the previous pose
-        for (int l = 0; l < p->levels; ++l)
-            if ((st = kf::launch_raycast(p->vol, nullptr, state, p->Ki[l], p->pv[l], p->pn[l], w >> l, h >> l, s)))
-                return st;
+        if ((st = kf::launch_raycast_levels(p->vol, nullptr, state, p->levels, p->Ki, p->pv, p->pn, w, h, s)))
+            return st;
         if (p->timing) KF_HIP(hipEventRecord(ev[2], s));
         // computePose (kinect_fusion.cpp:146-161)
         kf::IcpLevel lv[3];
@@ -182,9 +181,8 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
             lv[l] = kf::IcpLevel{p->vpyr[l], p->pv[l], p->pn[l], w >> l, h >> l, {0}};
             memcpy(lv[l].K, p->K[l], sizeof(lv[l].K));
         }
-        if ((st = kf::launch_icp_begin(state, nullptr, s))) return st;
+        if ((st = kf::launch_icp_begin(p->icp, nullptr, s))) return st;
         if ((st = kf::launch_icp_loop(p->icp, lv, p->levels, s))) return st;
-        if ((st = kf::launch_icp_commit(state, s))) return st;
         if (p->timing) KF_HIP(hipEventRecord(ev[3], s));
     }
     // fuseCurrentDepthToTSDF (kinect_fusion.cpp:179-183): unfiltered metres at the new pose

@@ -22,18 +22,35 @@ namespace kf {
 namespace {
 
 constexpr int kMaxMarchSteps = 1 << 20;
+// A sample whose 8 trilinear corners lie in one brick whose lower bound of F is
+// at least this value is >= 0.99 (convex weights; 1e-3 of margin covers the
+// rounding of the three lerp levels), so it can neither end the march nor
+// shorten the step.
+constexpr float kSkipBound = 0.991f;
 
-struct RaycastArgs {
-    const void* f;
-    int n, nb, format;
-    float width, mue;
-    float pose[16];
-    const PoseState* dyn;  // when set, the pose comes from dyn->pose
+struct RaycastLevel {
     float Kinv[9];
     float4* vout;
     float4* nout;
-    int w, h;
-    int tiles_x, tiles_y;
+    int w, h, tiles_x, tile0;  // tile0: first global tile of this level
+};
+
+// The fast skip test works on texel coordinates computed with a reciprocal
+// instead of the exact division (a few 1e-5 voxel apart): a sample counts as
+// inside a region only when its coordinate is at least kMargin voxel inside it,
+// so its exact stencil is inside that region too.
+constexpr float kMargin = 0.01f;
+
+struct RaycastArgs {
+    const void* f;
+    const float* brick_min;  // per-brick lower bound of F (null: no skipping)
+    const int* super_key;    // per-super-brick lower bound of F, order_key form
+    int n, nb, ns, format;
+    float width, mue;
+    float pose[16];
+    const PoseState* dyn;  // when set, the pose comes from dyn->pose
+    RaycastLevel lv[3];
+    int levels, tiles;
 };
 
 template <int FMT>
@@ -43,26 +60,42 @@ __device__ __forceinline__ float voxel_f(const void* f, int64_t q)
     return snorm_decode(static_cast<const int16_t*>(f)[q]);
 }
 
-// Trilinear sample at normalized coordinates (u, v, s).
-template <int FMT>
-__device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float u, float v, float s)
+// Texel coordinates of a trilinear sample at normalized (u, v, s):
+// x = u*n - 1/2, i0 = floor(x) clamped to [0, n-1], i1 = i0 + 1 clamped.
+struct Stencil {
+    int i0, j0, k0, i1, j1, k1;
+    float a, b, c;
+};
+__device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
 {
+    Stencil st;
     float x = u * (float)n - 0.5f, y = v * (float)n - 0.5f, z = s * (float)n - 0.5f;
     float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-    float a = x - fx, b = y - fy, c = z - fz;
+    st.a = x - fx;
+    st.b = y - fy;
+    st.c = z - fz;
     const float lo = -1.0f, hi = (float)n;
     fx = fx < lo ? lo : (fx > hi ? hi : fx);
     fy = fy < lo ? lo : (fy > hi ? hi : fy);
     fz = fz < lo ? lo : (fz > hi ? hi : fz);
     int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-    int i1 = clampi(i0 + 1, 0, n - 1), j1 = clampi(j0 + 1, 0, n - 1), k1 = clampi(k0 + 1, 0, n - 1);
-    i0 = clampi(i0, 0, n - 1);
-    j0 = clampi(j0, 0, n - 1);
-    k0 = clampi(k0, 0, n - 1);
-    float c000 = voxel_f<FMT>(F, brick_index(nb, i0, j0, k0)), c100 = voxel_f<FMT>(F, brick_index(nb, i1, j0, k0));
-    float c010 = voxel_f<FMT>(F, brick_index(nb, i0, j1, k0)), c110 = voxel_f<FMT>(F, brick_index(nb, i1, j1, k0));
-    float c001 = voxel_f<FMT>(F, brick_index(nb, i0, j0, k1)), c101 = voxel_f<FMT>(F, brick_index(nb, i1, j0, k1));
-    float c011 = voxel_f<FMT>(F, brick_index(nb, i0, j1, k1)), c111 = voxel_f<FMT>(F, brick_index(nb, i1, j1, k1));
+    st.i1 = clampi(i0 + 1, 0, n - 1);
+    st.j1 = clampi(j0 + 1, 0, n - 1);
+    st.k1 = clampi(k0 + 1, 0, n - 1);
+    st.i0 = clampi(i0, 0, n - 1);
+    st.j0 = clampi(j0, 0, n - 1);
+    st.k0 = clampi(k0, 0, n - 1);
+    return st;
+}
+
+template <int FMT>
+__device__ __forceinline__ float trilinear(const void* F, int nb, const Stencil& t)
+{
+    float c000 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k0)), c100 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k0));
+    float c010 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k0)), c110 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k0));
+    float c001 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k1)), c101 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k1));
+    float c011 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k1)), c111 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k1));
+    const float a = t.a, b = t.b, c = t.c;
     float e00 = (1.0f - a) * c000 + a * c100, e10 = (1.0f - a) * c010 + a * c110;
     float e01 = (1.0f - a) * c001 + a * c101, e11 = (1.0f - a) * c011 + a * c111;
     float g0 = (1.0f - b) * e00 + b * e10, g1 = (1.0f - b) * e01 + b * e11;
@@ -70,23 +103,69 @@ __device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float
 }
 
 template <int FMT>
+__device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float u, float v, float s)
+{
+    return trilinear<FMT>(F, nb, stencil(n, u, v, s));
+}
+
+// True when the sample is known to be >= 0.99 without reading the voxels.
+__device__ __forceinline__ bool skippable(const RaycastArgs& a, const Stencil& t)
+{
+    if (!a.brick_min) return false;
+    const int b0 = ((t.k0 >> 3) * a.nb + (t.j0 >> 3)) * a.nb + (t.i0 >> 3);
+    const int b1 = ((t.k1 >> 3) * a.nb + (t.j1 >> 3)) * a.nb + (t.i1 >> 3);
+    return b0 == b1 && a.brick_min[b0] >= kSkipBound;
+}
+
+// Conservative skip test on approximate texel coordinates (xa, ya, za): true
+// only if the exact stencil lies inside one super-brick (or one brick) whose
+// lower bound of F is >= kSkipBound.  The last super-brick's bound is cached
+// in (cs, cb), so a march through free space reads no memory at all.
+__device__ __forceinline__ bool fast_skip(const RaycastArgs& a, float xa, float ya, float za, int& cs, float& cb)
+{
+    const float hi = (float)a.n - 1.0f - kMargin;
+    if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return false;
+    const int xi = (int)xa, yi = (int)ya, zi = (int)za;
+    const int sx = xi >> 6, sy = yi >> 6, sz = zi >> 6;
+    const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
+    const float shi = (float)(kSuper - 1) - kMargin;
+    if (ux >= kMargin && uy >= kMargin && uz >= kMargin && ux < shi && uy < shi && uz < shi) {
+        const int sidx = (sz * a.ns + sy) * a.ns + sx;
+        if (sidx != cs) {
+            cs = sidx;
+            cb = order_value(a.super_key[sidx]);
+        }
+        if (cb >= kSkipBound) return true;
+    }
+    const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
+    const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
+    const float bhi = (float)(kBrick - 1) - kMargin;
+    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi)
+        return a.brick_min[(bz * a.nb + by) * a.nb + bx] >= kSkipBound;
+    return false;
+}
+
+template <int FMT>
 __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
 {
     // XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
     // contiguous band of tiles.
-    const int tiles = a.tiles_x * a.tiles_y;
     const int g = blockIdx.x;
-    const int per_xcd = (tiles + 7) / 8;
+    const int per_xcd = (a.tiles + 7) / 8;
     const int tile = (g % 8) * per_xcd + g / 8;
-    if (tile >= tiles) return;
-    const int i = (tile % a.tiles_x) * 8 + (threadIdx.x & 7);
-    const int j = (tile / a.tiles_x) * 8 + (threadIdx.x >> 3);
-    if (i >= a.w || j >= a.h) return;
+    if (tile >= a.tiles) return;
+    int l = 0;
+    while (l + 1 < a.levels && tile >= a.lv[l + 1].tile0) ++l;
+    const RaycastLevel& L = a.lv[l];
+    const int lt = tile - L.tile0;
+    const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
+    const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
+    if (i >= L.w || j >= L.h) return;
 
     const float* pose = a.dyn ? a.dyn->pose : a.pose;
     float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     float M[9];
-    m3m3(Rm, a.Kinv, M);
+    m3m3(Rm, L.Kinv, M);
     const v3 o = mk3(pose[12], pose[13], pose[14]);
     const float width = a.width, mue = a.mue, hw = width * 0.5f;
 
@@ -111,20 +190,42 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
     const float farv = (tmax < kMaxDistance) ? tmax : kMaxDistance;
     if (nearv >= farv) return;
 
-    // raycast.cu:75-117
+    // raycast.cu:75-117.  A skipped sample only sets prev; its value is
+    // computed if and when the zero-crossing formula needs it.
     const float large = mue * 1.99f, small = mue * 0.1f;
-    float step = large, prev = 0.0f, tstar = 0.0f;
+    float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f;
+    bool prev_skipped = false;
+    const float invw = 1.0f / width, nf = (float)a.n;
+    int cs = -1;
+    float cb = 0.0f;
     int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
     for (float t = nearv; t < farv; t = t + step) {
         if (++guard > kMaxMarchSteps) break;
         v3 p = add3(o, mul3s(dir, t));
-        float f = sample_tsdf<FMT>(a.f, a.n, a.nb, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+        if (a.brick_min && fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
+                                     (p.z * invw + 0.5f) * nf - 0.5f, cs, cb)) {
+            prev_skipped = true;
+            prev_t = t;
+            continue;
+        }
+        const Stencil st = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+        if (skippable(a, st)) {
+            prev_skipped = true;
+            prev_t = t;
+            continue;
+        }
+        float f = trilinear<FMT>(a.f, a.nb, st);
         if (f < 0.0f) {
             if (t != nearv) {
                 if (step == large) {
                     t = t - mue;
                     step = small;
                     continue;
+                }
+                if (prev_skipped) {
+                    v3 pp = add3(o, mul3s(dir, prev_t));
+                    prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
+                                            pp.z / width + 0.5f);
                 }
                 tstar = t - step * prev / (f - prev);
             }
@@ -133,9 +234,10 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
             step = small;
         }
         prev = f;
+        prev_skipped = false;
     }
 
-    const int64_t q = (int64_t)j * a.w + i;
+    const int64_t q = (int64_t)j * L.w + i;
     if (tstar != 0.0f) {
         v3 vtx = add3(o, mul3s(dir, tstar));
         // raycast.cu:29-42
@@ -151,37 +253,47 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
         float gn = len3(g);
         if (is_depth_valid(gn)) {
             v3 nn = div3s(g, gn);
-            a.vout[q] = make_float4(vtx.x, vtx.y, vtx.z, kValid);
-            a.nout[q] = make_float4(nn.x, nn.y, nn.z, kValid);
+            L.vout[q] = make_float4(vtx.x, vtx.y, vtx.z, kValid);
+            L.nout[q] = make_float4(nn.x, nn.y, nn.z, kValid);
             return;
         }
     }
-    a.vout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
-    a.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+    L.vout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+    L.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
 }
 
 }  // namespace
 
-int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
-                   float4* vout, float4* nout, int w, int h, hipStream_t s)
+int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState* dyn_state, int levels,
+                          const float (*inv_k)[9], float4* const* vout, float4* const* nout, int w0, int h0,
+                          hipStream_t s)
 {
     RaycastArgs a{};
     a.f = v->f;
+    a.brick_min = v->brick_min;
+    a.super_key = v->super_key;
     a.n = v->n;
     a.nb = v->nb;
+    a.ns = v->ns;
     a.format = v->format;
     a.width = v->width;
     a.mue = v->mue;
     if (pose) memcpy(a.pose, pose, sizeof(a.pose));
     a.dyn = dyn_state;
-    memcpy(a.Kinv, inv_k, sizeof(a.Kinv));
-    a.vout = vout;
-    a.nout = nout;
-    a.w = w;
-    a.h = h;
-    a.tiles_x = (w + 7) / 8;
-    a.tiles_y = (h + 7) / 8;
-    const int tiles = a.tiles_x * a.tiles_y;
+    a.levels = levels;
+    int tiles = 0;
+    for (int l = 0; l < levels; ++l) {
+        RaycastLevel& L = a.lv[l];
+        memcpy(L.Kinv, inv_k[l], sizeof(L.Kinv));
+        L.vout = vout[l];
+        L.nout = nout[l];
+        L.w = w0 >> l;
+        L.h = h0 >> l;
+        L.tiles_x = (L.w + 7) / 8;
+        L.tile0 = tiles;
+        tiles += L.tiles_x * ((L.h + 7) / 8);
+    }
+    a.tiles = tiles;
     const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
     if (v->format == KF_VOXEL_F32)
         raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
@@ -189,6 +301,14 @@ int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_s
         raycast_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(a);
     KF_LAUNCHED("raycast_kernel");
     return KF_OK;
+}
+
+int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
+                   float4* vout, float4* nout, int w, int h, hipStream_t s)
+{
+    float ki[1][9];
+    memcpy(ki[0], inv_k, sizeof(ki[0]));
+    return launch_raycast_levels(v, pose, dyn_state, 1, ki, &vout, &nout, w, h, s);
 }
 
 }  // namespace kf

@@ -4,9 +4,10 @@
 // HBM layout: two planes (f, w) of n^3 voxels in 8^3 bricks (kf_device.hpp).
 //   KF_VOXEL_F32        : f float, w float          (8 B / voxel)
 //   KF_VOXEL_SNORM16_U8 : f snorm16, w uint8 count  (3 B / voxel)
-// One workgroup of 256 lanes owns one brick and fuses its 512 voxels (two per
-// lane, 4 z-slices apart), so every plane access of a wave is a contiguous
-// 256 B (f32) run.  Bricks whose bounding sphere lies outside the view frustum
+// A cull pass lists the bricks whose bounding sphere meets the view frustum;
+// then one wave fuses one listed brick at a time, 8 voxels per lane (one per
+// z-slice), so every plane access of a wave is a contiguous 256 B (f32) run and
+// the listed bricks are spread evenly over the waves.  Bricks whose bounding sphere lies outside the view frustum
 // are skipped before any voxel is visited; the test is conservative, so the
 // result is identical to visiting every voxel.
 #include <stdio.h>
@@ -20,6 +21,8 @@
 #include "kf_kernels.hpp"
 
 namespace kf {
+
+constexpr unsigned kFuseBlocks = 2048;  // fuse workgroups: 256 CUs x 8 resident workgroups of 4 waves
 
 namespace {
 
@@ -35,6 +38,9 @@ struct FuseArgs {
     void* f;
     void* wt;
     unsigned long long* touched;
+    float* brick_min;
+    int* super_key;
+    int ns;
     int cull;  // 1 when K is a plain pinhole matrix (culling planes valid)
 };
 
@@ -59,77 +65,174 @@ __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip
     return true;
 }
 
-// tsdf.cu:20-75 for one voxel; returns true when the voxel was updated.
+// tsdf.cu:20-75 for the 8 voxels of one brick column (lane = (j, i) of the
+// brick, kl = 0..7 along z).  The projection of all 8 voxels is computed first
+// and their depth and voxel loads are issued together, so a lane has up to 8
+// independent memory requests in flight.  Returns the number of updated voxels.
 template <int FMT>
-__device__ __forceinline__ bool fuse_voxel(const FuseArgs& a, const float* ip, v3 col /* ip0*x + ip1*y per row */,
-                                           float z, int64_t q)
+__device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v3 col, const float* zk, int64_t base,
+                                           float& fmin)
 {
-    // pe = (m0*x + m1*y) + (m2*z + m3*1): the first pair is shared by the column
-    v3 pe = mk3(col.x + (ip[8] * z + ip[12] * 1.0f), col.y + (ip[9] * z + ip[13] * 1.0f),
-                col.z + (ip[10] * z + ip[14] * 1.0f));
-    if (!(pe.z >= kMinDistance && pe.z < kMaxDistance)) return false;
-    v3 pix = m3v(a.K, pe);
-    float px = pix.x / pix.z, py = pix.y / pix.z;
-    if (!(px >= 0.0f && px < (float)a.w && py >= 0.0f && py < (float)a.h)) return false;
-    float d = a.depth[(int64_t)(int)py * a.w + (int)px];
-    if (!is_depth_valid(d)) return false;
-    float diff = d - pe.z;
-    if (!(diff >= -a.mue)) return false;
-    float f = diff / a.mue;
-    f = (f < 1.0f) ? f : 1.0f;
-    if (FMT == KF_VOXEL_F32) {
-        float* F = static_cast<float*>(a.f);
-        float* Wt = static_cast<float*>(a.wt);
-        float wt = 1.0f / d;
-        float Fv = F[q], Wv = Wt[q];
-        Fv = (Fv * Wv + f * wt) / (Wv + wt);
-        Wv = Wv + wt;
-        F[q] = Fv;
-        Wt[q] = Wv;
-    } else {
-        // precision form: unit weight per observation, count capped at 128
-        int16_t* F = static_cast<int16_t*>(a.f);
-        uint8_t* Wt = static_cast<uint8_t*>(a.wt);
-        float Wv = (float)Wt[q];
-        float Fv = snorm_decode(F[q]);
-        Fv = (Fv * Wv + f) / (Wv + 1.0f);
-        F[q] = snorm_encode(Fv);
-        Wt[q] = (uint8_t)(Wv < 128.0f ? Wv + 1.0f : 128.0f);
+    float dep[8], pz[8];
+    int pix[8];
+    bool hit[8];
+#pragma unroll
+    for (int kl = 0; kl < 8; ++kl) {
+        const float z = zk[kl];
+        // pe = (m0*x + m1*y) + (m2*z + m3*1): the first pair is shared by the column
+        v3 pe = mk3(col.x + (ip[8] * z + ip[12] * 1.0f), col.y + (ip[9] * z + ip[13] * 1.0f),
+                    col.z + (ip[10] * z + ip[14] * 1.0f));
+        bool ok = pe.z >= kMinDistance && pe.z < kMaxDistance;
+        int q = 0;
+        if (ok) {
+            v3 pv = m3v(a.K, pe);
+            float px = pv.x / pv.z, py = pv.y / pv.z;
+            ok = px >= 0.0f && px < (float)a.w && py >= 0.0f && py < (float)a.h;
+            if (ok) q = (int)py * a.w + (int)px;
+        }
+        hit[kl] = ok;
+        pix[kl] = q;
+        pz[kl] = pe.z;
     }
-    return true;
+#pragma unroll
+    for (int kl = 0; kl < 8; ++kl) dep[kl] = hit[kl] ? a.depth[pix[kl]] : 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int kl = 0; kl < 8; ++kl) {
+        const float d = dep[kl];
+        const float diff = d - pz[kl];
+        if (!(hit[kl] && is_depth_valid(d) && diff >= -a.mue)) continue;
+        float f = diff / a.mue;
+        f = (f < 1.0f) ? f : 1.0f;
+        const int64_t q = base + 64 * kl;
+        if (FMT == KF_VOXEL_F32) {
+            float* F = static_cast<float*>(a.f);
+            float* Wt = static_cast<float*>(a.wt);
+            float wt = 1.0f / d;
+            float Fv = F[q], Wv = Wt[q];
+            Fv = (Fv * Wv + f * wt) / (Wv + wt);
+            Wv = Wv + wt;
+            F[q] = Fv;
+            Wt[q] = Wv;
+            fmin = fminf(fmin, Fv);
+        } else {
+            // precision form: unit weight per observation, count capped at 128
+            int16_t* F = static_cast<int16_t*>(a.f);
+            uint8_t* Wt = static_cast<uint8_t*>(a.wt);
+            float Wv = (float)Wt[q];
+            float Fv = snorm_decode(F[q]);
+            Fv = (Fv * Wv + f) / (Wv + 1.0f);
+            const int16_t qf = snorm_encode(Fv);
+            F[q] = qf;
+            Wt[q] = (uint8_t)(Wv < 128.0f ? Wv + 1.0f : 128.0f);
+            fmin = fminf(fmin, snorm_decode(qf));
+        }
+        ++cnt;
+    }
+    return cnt;
 }
 
-template <int FMT>
-__global__ __launch_bounds__(256) void fuse_kernel(FuseArgs a)
+__device__ __forceinline__ void load_inv_pose(const FuseArgs& a, float* ip)
+{
+    if (threadIdx.x < 16) ip[threadIdx.x] = a.dyn ? a.dyn->inv_pose[threadIdx.x] : a.inv_pose[threadIdx.x];
+    __syncthreads();
+}
+
+// Pass 1: one lane per brick; bricks whose bounding sphere meets the view
+// frustum are appended to `list`, one atomic per workgroup.  The list length
+// lives in count[parity]; this pass also clears count[1 - parity], which the
+// previous launch used, for the next launch.
+constexpr int kCullThreads = 1024;
+
+__global__ __launch_bounds__(kCullThreads) void fuse_cull_kernel(FuseArgs a, int64_t nbricks, int* list,
+                                                                 unsigned int* count, int parity)
 {
     if (a.dyn && a.dyn->frame_skip_fuse) return;
     __shared__ float ip[16];
-    if (threadIdx.x < 16) ip[threadIdx.x] = a.dyn ? a.dyn->inv_pose[threadIdx.x] : a.inv_pose[threadIdx.x];
-    __syncthreads();
-    const int64_t brick = blockIdx.x;
-    const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb), bk = (int)(brick / ((int64_t)a.nb * a.nb));
-    if (a.cull && !brick_visible(a, ip, bi, bj, bk)) return;
-    const int lane = threadIdx.x;
-    const int i = bi * 8 + (lane & 7), j = bj * 8 + ((lane >> 3) & 7), kl = lane >> 6;
-    const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
-    const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
-    int cnt = 0;
-#pragma unroll
-    for (int hpart = 0; hpart < 2; ++hpart) {
-        int k = bk * 8 + kl + 4 * hpart;
-        cnt += fuse_voxel<FMT>(a, ip, col, a.zk[k], brick * kBrickVoxels + lane + 256 * hpart) ? 1 : 0;
+    __shared__ unsigned int wave_base[kCullThreads / 64 + 1];
+    load_inv_pose(a, ip);
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[1 - parity] = 0u;
+    const int64_t brick = (int64_t)blockIdx.x * kCullThreads + threadIdx.x;
+    bool vis = false;
+    if (brick < nbricks) {
+        const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
+        const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+        vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
     }
-    // wave sum of the touched counts, one atomic per wave
+    const uint64_t m = __ballot(vis);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_base[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int total = 0;
+        for (int w = 0; w < kCullThreads / 64; ++w) {
+            unsigned int c = wave_base[w];
+            wave_base[w] = total;
+            total += c;
+        }
+        wave_base[kCullThreads / 64] = total ? atomicAdd(&count[parity], total) : 0u;
+    }
+    __syncthreads();
+    if (vis)
+        list[wave_base[kCullThreads / 64] + wave_base[wave] + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] =
+            (int)brick;
+}
+
+// Pass 2: waves stride over the visible bricks.  Each workgroup adds its count
+// of updated voxels to its own slot of `touched` (no atomics: launches on one
+// stream never run concurrently, and slot b belongs to workgroup b).
+constexpr int kFuseWaves = 4;
+
+template <int FMT>
+__global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, const int* list,
+                                                                    const unsigned int* count)
+{
+    if (a.dyn && a.dyn->frame_skip_fuse) return;
+    __shared__ float ip[16];
+    __shared__ int wave_cnt[kFuseWaves];
+    load_inv_pose(a, ip);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned int total = *count;
+    const unsigned int waves = gridDim.x * kFuseWaves;
+    int cnt = 0;
+    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves) {
+        const int64_t brick = list[item];
+        const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
+        const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+        const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
+        const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
+        const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
+        float fmin = __int_as_float(0x7f800000);
+        cnt += fuse_column<FMT>(a, ip, col, a.zk + bk * 8, brick * kBrickVoxels + lane, fmin);
+        // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) fmin = fminf(fmin, __shfl_xor(fmin, off, 64));
+        if (lane == 0 && fmin < a.brick_min[brick]) {
+            a.brick_min[brick] = fmin;
+            const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
+            atomicMin(&a.super_key[sidx], order_key(fmin));
+        }
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(a.touched, (unsigned long long)cnt);
+    if (lane == 0) wave_cnt[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int w = 0; w < kFuseWaves; ++w) c += wave_cnt[w];
+        if (c) a.touched[blockIdx.x] += (unsigned long long)c;
+    }
 }
 
 template <int FMT>
-__global__ __launch_bounds__(256) void init_kernel(void* f, void* wt, int64_t count, float f0, float w0)
+__global__ __launch_bounds__(256) void init_kernel(void* f, void* wt, int64_t count, float f0, float w0,
+                                                   float* brick_min, int64_t nbricks, int* super_key, int nsuper)
 {
     int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * 256;
+    const float fb = (FMT == KF_VOXEL_F32) ? f0 : snorm_decode(snorm_encode(f0));
+    for (int64_t b = q; b < nbricks; b += stride) brick_min[b] = fb;
+    for (int64_t b = q; b < nsuper; b += stride) super_key[b] = order_key(fb);
     for (; q < count; q += stride) {
         if (FMT == KF_VOXEL_F32) {
             static_cast<float*>(f)[q] = f0;
@@ -180,6 +283,36 @@ __global__ __launch_bounds__(256) void volume_to_slab(float* __restrict__ slab, 
     }
 }
 
+// Exact per-brick minimum of F (after an upload): one wave per brick.
+__global__ void super_reset_kernel(int* super_key, int nsuper)
+{
+    int q = blockIdx.x * 256 + threadIdx.x;
+    if (q < nsuper) super_key[q] = 0x7f800000;  // order_key(+inf)
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void brick_min_kernel(const void* f, float* brick_min, int64_t nbricks, int nb,
+                                                        int ns, int* super_key)
+{
+    const int64_t brick = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (brick >= nbricks) return;
+    const int lane = threadIdx.x & 63;
+    float m = __int_as_float(0x7f800000);
+    for (int q = lane; q < kBrickVoxels; q += 64) {
+        const int64_t idx = brick * kBrickVoxels + q;
+        const float v = (FMT == KF_VOXEL_F32) ? static_cast<const float*>(f)[idx]
+                                              : snorm_decode(static_cast<const int16_t*>(f)[idx]);
+        m = fminf(m, v);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
+    if (lane == 0) {
+        brick_min[brick] = m;
+        const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / ((int64_t)nb * nb));
+        atomicMin(&super_key[((bk >> 3) * ns + (bj >> 3)) * ns + (bi >> 3)], order_key(m));
+    }
+}
+
 bool is_pinhole(const float* K)
 {
     return K[1] == 0.0f && K[2] == 0.0f && K[3] == 0.0f && K[5] == 0.0f && K[8] == 1.0f && K[0] > 0.0f &&
@@ -219,6 +352,18 @@ int for_each_slab(const kf_volume* v, bool to_volume, Fn&& host_side)
         }
     }
     (void)hipFree(dslab);
+    if (st == KF_OK && to_volume) {
+        const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
+        const unsigned b = (unsigned)((nbricks + 3) / 4);
+        const int nsuper = v->ns * v->ns * v->ns;
+        super_reset_kernel<<<(nsuper + 255) / 256, 256>>>(v->super_key, nsuper);
+        if (v->format == KF_VOXEL_F32)
+            brick_min_kernel<KF_VOXEL_F32><<<b, 256>>>(v->f, v->brick_min, nbricks, v->nb, v->ns, v->super_key);
+        else
+            brick_min_kernel<KF_VOXEL_SNORM16_U8><<<b, 256>>>(v->f, v->brick_min, nbricks, v->nb, v->ns, v->super_key);
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) st = fail_hip(e, "brick_min_kernel");
+    }
     return st;
 }
 
@@ -243,13 +388,24 @@ int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, 
     a.f = v->f;
     a.wt = v->w;
     a.touched = v->touched;
+    a.brick_min = v->brick_min;
+    a.super_key = v->super_key;
+    a.ns = v->ns;
     a.cull = is_pinhole(K) ? 1 : 0;
-    const unsigned blocks = (unsigned)((int64_t)v->nb * v->nb * v->nb);
+    const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
+    const int parity = (int)(v->fuse_launches & 1);
+    fuse_cull_kernel<<<(unsigned)((nbricks + kCullThreads - 1) / kCullThreads), kCullThreads, 0, s>>>(
+        a, nbricks, v->brick_list, v->list_count, parity);
+    KF_LAUNCHED("fuse_cull_kernel");
+    // 256 CUs x 8 resident workgroups of 4 waves; fewer when the volume is small
+    const unsigned blocks = (unsigned)std::min<int64_t>((nbricks + kFuseWaves - 1) / kFuseWaves, kFuseBlocks);
     if (v->format == KF_VOXEL_F32)
-        fuse_kernel<KF_VOXEL_F32><<<blocks, 256, 0, s>>>(a);
+        fuse_list_kernel<KF_VOXEL_F32><<<blocks, 64 * kFuseWaves, 0, s>>>(a, v->brick_list, v->list_count + parity);
     else
-        fuse_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 256, 0, s>>>(a);
-    KF_LAUNCHED("fuse_kernel");
+        fuse_list_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64 * kFuseWaves, 0, s>>>(a, v->brick_list,
+                                                                                 v->list_count + parity);
+    KF_LAUNCHED("fuse_list_kernel");
+    v->fuse_launches += 1;
     return KF_OK;
 }
 
@@ -257,10 +413,14 @@ int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s)
 {
     int64_t count = (int64_t)v->n * v->n * v->n;
     unsigned blocks = (unsigned)std::min<int64_t>((count + 255) / 256, 65536);
+    const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
+    const int nsuper = v->ns * v->ns * v->ns;
     if (v->format == KF_VOXEL_F32)
-        init_kernel<KF_VOXEL_F32><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt);
+        init_kernel<KF_VOXEL_F32><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_min, nbricks, v->super_key,
+                                                         nsuper);
     else
-        init_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt);
+        init_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_min, nbricks,
+                                                                v->super_key, nsuper);
     KF_LAUNCHED("init_kernel");
     return KF_OK;
 }
@@ -284,6 +444,7 @@ int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** 
     kf_volume* v = new kf_volume();
     v->n = n;
     v->nb = n / 8;
+    v->ns = (v->nb + 7) / 8;
     v->format = format;
     v->width = width_m;
     v->res = width_m / (float)(n - 1);  // voxel_grid.h:43
@@ -294,7 +455,13 @@ int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** 
     hipError_t e;
     if ((e = hipMalloc(&v->f, (size_t)count * fb)) != hipSuccess || (e = hipMalloc(&v->w, (size_t)count * wb)) != hipSuccess ||
         (e = hipMalloc(&v->zk, sizeof(float) * n)) != hipSuccess ||
-        (e = hipMalloc(&v->touched, sizeof(unsigned long long))) != hipSuccess) {
+        (e = hipMalloc(&v->touched, sizeof(unsigned long long) * kf::kFuseBlocks)) != hipSuccess ||
+        (e = hipMemset(v->touched, 0, sizeof(unsigned long long) * kf::kFuseBlocks)) != hipSuccess ||
+        (e = hipMalloc(&v->brick_list, sizeof(int) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
+        (e = hipMalloc(&v->brick_min, sizeof(float) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
+        (e = hipMalloc(&v->super_key, sizeof(int) * (size_t)v->ns * v->ns * v->ns)) != hipSuccess ||
+        (e = hipMalloc(&v->list_count, 2 * sizeof(unsigned int))) != hipSuccess ||
+        (e = hipMemset(v->list_count, 0, 2 * sizeof(unsigned int))) != hipSuccess) {
         kf_volume_destroy(v);
         return kf::fail_hip(e, "kf_volume_create");
     }
@@ -305,9 +472,7 @@ int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** 
         z = z + v->res;
         zk[k] = z;
     }
-    unsigned long long zero = 0;
-    if ((e = hipMemcpy(v->zk, zk.data(), sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(v->touched, &zero, sizeof(zero), hipMemcpyHostToDevice)) != hipSuccess) {
+    if ((e = hipMemcpy(v->zk, zk.data(), sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) {
         kf_volume_destroy(v);
         return kf::fail_hip(e, "kf_volume_create");
     }
@@ -336,6 +501,10 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->w);
     (void)hipFree(v->zk);
     (void)hipFree(v->touched);
+    (void)hipFree(v->brick_list);
+    (void)hipFree(v->brick_min);
+    (void)hipFree(v->super_key);
+    (void)hipFree(v->list_count);
     delete v;
     return KF_OK;
 }
@@ -361,9 +530,11 @@ int kf_volume_bytes(const kf_volume* v, size_t* bytes)
 int kf_volume_touched_total(const kf_volume* v, int64_t* count)
 {
     if (!v || !count) return kf::fail(KF_ERR_ARG, "kf_volume_touched_total: null argument");
-    unsigned long long c = 0;
+    std::vector<unsigned long long> part(kf::kFuseBlocks);
     KF_HIP(hipDeviceSynchronize());
-    KF_HIP(hipMemcpy(&c, v->touched, sizeof(c), hipMemcpyDeviceToHost));
+    KF_HIP(hipMemcpy(part.data(), v->touched, sizeof(unsigned long long) * part.size(), hipMemcpyDeviceToHost));
+    unsigned long long c = 0;
+    for (auto x : part) c += x;
     *count = (int64_t)c;
     return KF_OK;
 }

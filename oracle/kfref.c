@@ -352,8 +352,24 @@ void KFO(initialize_grid)(real* F, real* W, int n, real f0, real w0)
  * Returns the number of updated ("touched") voxels.
  * Pinned by: the fronto-parallel wall closed form, repeat-invariance of F and
  * W = k/D after k identical frames (tests/test_oracle_tsdf.py). */
-int64_t KFO(fuse)(real* F, real* W, int n, real width, real mue,
-                  const real* depth, int w, int h, const real* inv_pose, const real* K)
+/* Precision form (snorm16 F, uint8 W): F is stored as q = rint(clamp(f, -1, 1) * 32767)
+ * and read as q / 32767; W counts observations: F = (F*W + f) / (W + 1),
+ * W = min(W + 1, 128).  Same projection and tests as the FP32 form. */
+#define KF_SNORM R(32767.0)
+static inline real snorm_decode(int16_t q) { return (real)q * (R(1.0) / KF_SNORM); }
+static inline int16_t snorm_encode(real f)
+{
+    real s = f * KF_SNORM;
+    s = FMIN(FMAX(s, -KF_SNORM), KF_SNORM);
+#ifdef KFREF_F64
+    return (int16_t)nearbyint(s);
+#else
+    return (int16_t)nearbyintf(s);
+#endif
+}
+
+static int64_t fuse_impl(void* Fp, void* Wp, int fmt, int n, real width, real mue,
+                         const real* depth, int w, int h, const real* inv_pose, const real* K)
 {
     real res = width / (real)(n - 1);
     real half = -width * R(0.5);
@@ -380,13 +396,25 @@ int64_t KFO(fuse)(real* F, real* W, int n, real width, real mue,
                 if (diff >= -mue) {
                     real f = diff / mue;
                     f = (f < R(1.0)) ? f : R(1.0);   /* glm::min(1, f) = (f < 1) ? f : 1 */
-                    real wt = R(1.0) / d;
                     int64_t q = ((int64_t)k * n + j) * n + i;
-                    real Fv = F[q], Wv = W[q];
-                    Fv = (Fv * Wv + f * wt) / (Wv + wt);
-                    Wv = Wv + wt;
-                    F[q] = Fv;
-                    W[q] = Wv;
+                    if (fmt == 0) {
+                        real* F = (real*)Fp;
+                        real* W = (real*)Wp;
+                        real wt = R(1.0) / d;
+                        real Fv = F[q], Wv = W[q];
+                        Fv = (Fv * Wv + f * wt) / (Wv + wt);
+                        Wv = Wv + wt;
+                        F[q] = Fv;
+                        W[q] = Wv;
+                    } else {
+                        int16_t* F = (int16_t*)Fp;
+                        uint8_t* W = (uint8_t*)Wp;
+                        real Wv = (real)W[q];
+                        real Fv = snorm_decode(F[q]);
+                        Fv = (Fv * Wv + f) / (Wv + R(1.0));
+                        F[q] = snorm_encode(Fv);
+                        W[q] = (uint8_t)(Wv < R(128.0) ? Wv + R(1.0) : R(128.0));
+                    }
                     ++touched;
                 }
             }
@@ -394,6 +422,25 @@ int64_t KFO(fuse)(real* F, real* W, int n, real width, real mue,
     free(zk);
     return touched;
 }
+int64_t KFO(fuse)(real* F, real* W, int n, real width, real mue, const real* depth, int w, int h,
+                  const real* inv_pose, const real* K)
+{
+    return fuse_impl(F, W, 0, n, width, mue, depth, w, h, inv_pose, K);
+}
+#ifndef KFREF_F64
+int64_t kfo_fuse_q(int16_t* F, uint8_t* W, int n, float width, float mue, const float* depth, int w, int h,
+                   const float* inv_pose, const float* K)
+{
+    return fuse_impl(F, W, 1, n, width, mue, depth, w, h, inv_pose, K);
+}
+void kfo_initialize_grid_q(int16_t* F, uint8_t* W, int n, float f0, float w0)
+{
+    int64_t cnt = (int64_t)n * n * n;
+    int16_t fq = snorm_encode(f0);
+    float wr = nearbyintf(fminf(fmaxf(w0, 0.0f), 255.0f));
+    for (int64_t q = 0; q < cnt; ++q) { F[q] = fq; W[q] = (uint8_t)wr; }
+}
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* Trilinear sample of the f plane at normalized coordinates (u,v,s), texel
@@ -407,7 +454,11 @@ int64_t KFO(fuse)(real* F, real* W, int n, real width, real mue,
  * as nested (1-t)*p + t*q along x, then y, then z. */
 static inline real lerp_(real p, real q, real t) { return (R(1.0) - t) * p + t * q; }
 static inline int64_t vidx(int n, int i, int j, int k) { return ((int64_t)k * n + j) * n + i; }
-real KFO(sample_tsdf)(const real* F, int n, real u, real v, real s)
+static inline real voxel_value(const void* F, int fmt, int64_t q)
+{
+    return fmt == 0 ? ((const real*)F)[q] : snorm_decode(((const int16_t*)F)[q]);
+}
+static real sample_any(const void* F, int fmt, int n, real u, real v, real s)
 {
     real x = u * (real)n - R(0.5), y = v * (real)n - R(0.5), z = s * (real)n - R(0.5);
     real fx = FLOOR(x), fy = FLOOR(y), fz = FLOOR(z);
@@ -420,15 +471,16 @@ real KFO(sample_tsdf)(const real* F, int n, real u, real v, real s)
     int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
     int i1 = clampi(i0 + 1, 0, n - 1), j1 = clampi(j0 + 1, 0, n - 1), k1 = clampi(k0 + 1, 0, n - 1);
     i0 = clampi(i0, 0, n - 1); j0 = clampi(j0, 0, n - 1); k0 = clampi(k0, 0, n - 1);
-    real c000 = F[vidx(n, i0, j0, k0)], c100 = F[vidx(n, i1, j0, k0)];
-    real c010 = F[vidx(n, i0, j1, k0)], c110 = F[vidx(n, i1, j1, k0)];
-    real c001 = F[vidx(n, i0, j0, k1)], c101 = F[vidx(n, i1, j0, k1)];
-    real c011 = F[vidx(n, i0, j1, k1)], c111 = F[vidx(n, i1, j1, k1)];
+    real c000 = voxel_value(F, fmt, vidx(n, i0, j0, k0)), c100 = voxel_value(F, fmt, vidx(n, i1, j0, k0));
+    real c010 = voxel_value(F, fmt, vidx(n, i0, j1, k0)), c110 = voxel_value(F, fmt, vidx(n, i1, j1, k0));
+    real c001 = voxel_value(F, fmt, vidx(n, i0, j0, k1)), c101 = voxel_value(F, fmt, vidx(n, i1, j0, k1));
+    real c011 = voxel_value(F, fmt, vidx(n, i0, j1, k1)), c111 = voxel_value(F, fmt, vidx(n, i1, j1, k1));
     real e00 = lerp_(c000, c100, a), e10 = lerp_(c010, c110, a);
     real e01 = lerp_(c001, c101, a), e11 = lerp_(c011, c111, a);
     real g0 = lerp_(e00, e10, b), g1 = lerp_(e01, e11, b);
     return lerp_(g0, g1, c);
 }
+real KFO(sample_tsdf)(const real* F, int n, real u, real v, real s) { return sample_any(F, 0, n, u, v, s); }
 
 /* raycast.cu:7-27 slab test with 1/dir. */
 static inline void intersect_bbox(v3 o, v3 inv, real lo, real hi, real* tmin_out, real* tmax_out)
@@ -448,16 +500,16 @@ static inline void intersect_bbox(v3 o, v3 inv, real lo, real hi, real* tmin_out
 
 /* raycast.cu:29-42: central difference at +-mue/2 (normalized offset
  * mue*0.5/width), returns (F(-) - F(+)) / offset, i.e. -grad F scaled. */
-static inline v3 compute_gradient(const real* F, int n, real width, real mue, v3 p)
+static inline v3 compute_gradient(const void* F, int fmt, int n, real width, real mue, v3 p)
 {
     real r = mue * R(0.5) / width;
     real u = p.x / width + R(0.5), v = p.y / width + R(0.5), s = p.z / width + R(0.5);
-    real fx0 = KFO(sample_tsdf)(F, n, u + r, v, s);
-    real fy0 = KFO(sample_tsdf)(F, n, u, v + r, s);
-    real fz0 = KFO(sample_tsdf)(F, n, u, v, s + r);
-    real fx = KFO(sample_tsdf)(F, n, u - r, v, s);
-    real fy = KFO(sample_tsdf)(F, n, u, v - r, s);
-    real fz = KFO(sample_tsdf)(F, n, u, v, s - r);
+    real fx0 = sample_any(F, fmt, n, u + r, v, s);
+    real fy0 = sample_any(F, fmt, n, u, v + r, s);
+    real fz0 = sample_any(F, fmt, n, u, v, s + r);
+    real fx = sample_any(F, fmt, n, u - r, v, s);
+    real fy = sample_any(F, fmt, n, u, v - r, s);
+    real fz = sample_any(F, fmt, n, u, v, s - r);
     return div3s(mk3(fx - fx0, fy - fy0, fz - fz0), r);
 }
 
@@ -472,8 +524,8 @@ static inline v3 compute_gradient(const real* F, int n, real width, real mue, v3
  *   "depth" (> 0.01, finite), else (0,0,0,-2) to both maps.
  * Pinned by: analytic plane and sphere TSDFs (hit depth, +-axis normals) and
  * the stale-pixel rule (tests/test_oracle_raycast.py). */
-void KFO(raycast)(const real* F, int n, real width, real mue, const real* pose, const real* Ki,
-                  int w, int h, real4* vout, real4* nout)
+static void raycast_impl(const void* F, int fmt, int n, real width, real mue, const real* pose, const real* Ki,
+                         int w, int h, real4* vout, real4* nout)
 {
     real Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     real M[9];
@@ -497,7 +549,7 @@ void KFO(raycast)(const real* F, int n, real width, real mue, const real* pose, 
             for (real t = nearv; t < farv; t = t + step) {
                 if (++guard > (1 << 20)) break;
                 v3 p = add3(o, mul3s(dir, t));
-                real f = KFO(sample_tsdf)(F, n, p.x / width + R(0.5), p.y / width + R(0.5), p.z / width + R(0.5));
+                real f = sample_any(F, fmt, n, p.x / width + R(0.5), p.y / width + R(0.5), p.z / width + R(0.5));
                 if (f < R(0.0)) {
                     if (t != nearv) {
                         if (step == large) {
@@ -517,7 +569,7 @@ void KFO(raycast)(const real* F, int n, real width, real mue, const real* pose, 
             real4 inval = {0, 0, 0, KF_INVALID};
             if (tstar != R(0.0)) {
                 v3 vtx = add3(o, mul3s(dir, tstar));
-                v3 g = compute_gradient(F, n, width, mue, vtx);
+                v3 g = compute_gradient(F, fmt, n, width, mue, vtx);
                 real gn = len3(g);
                 if (is_depth_valid(gn)) {
                     real4 a = {vtx.x, vtx.y, vtx.z, KF_VALID};
@@ -532,6 +584,18 @@ void KFO(raycast)(const real* F, int n, real width, real mue, const real* pose, 
             nout[q] = inval;
         }
 }
+void KFO(raycast)(const real* F, int n, real width, real mue, const real* pose, const real* Ki, int w, int h,
+                  real4* vout, real4* nout)
+{
+    raycast_impl(F, 0, n, width, mue, pose, Ki, w, h, vout, nout);
+}
+#ifndef KFREF_F64
+void kfo_raycast_q(const int16_t* F, int n, float width, float mue, const float* pose, const float* Ki, int w, int h,
+                   kfo_float4* vout, kfo_float4* nout)
+{
+    raycast_impl(F, 1, n, width, mue, pose, Ki, w, h, vout, nout);
+}
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* icp.cuh:40-49: pixel of s in the previous frame, (K * R_prev^T) * (s - t_prev),
@@ -880,6 +944,9 @@ struct KFO(pipeline) {
     int iters[3];
     real K[3][9], Ki[3][9];
     real pose[16], inv_pose[16];
+    int fmt;          /* 0: FP32 F/W, 1: snorm16 F + uint8 W (FP32 build only) */
+    int16_t* Fq;
+    uint8_t* Wq;
     real *F, *W, *raw_m, *dpyr[3];
     real4 *vpyr[3], *pv[3], *pn[3];
     int frame;
@@ -939,7 +1006,7 @@ pipe_t* KFO(pipeline_create)(int w, int h, int n, real width, real mue, real fx,
 void KFO(pipeline_destroy)(pipe_t* p)
 {
     if (!p) return;
-    free(p->F); free(p->W); free(p->raw_m);
+    free(p->F); free(p->W); free(p->Fq); free(p->Wq); free(p->raw_m);
     for (int l = 0; l < 3; ++l) { free(p->dpyr[l]); free(p->vpyr[l]); free(p->pv[l]); free(p->pn[l]); }
     free(p);
 }
@@ -960,7 +1027,8 @@ int KFO(pipeline_frame)(pipe_t* p, const uint16_t* raw)
         double t1 = now_ms();
         p->stage_ms[0] += t1 - t0;
         for (int l = 0; l < p->levels; ++l)
-            KFO(raycast)(p->F, p->n, p->width, p->mue, p->pose, p->Ki[l], w >> l, h >> l, p->pv[l], p->pn[l]);
+            raycast_impl(p->fmt ? (const void*)p->Fq : (const void*)p->F, p->fmt, p->n, p->width, p->mue, p->pose,
+                         p->Ki[l], w >> l, h >> l, p->pv[l], p->pn[l]);
         double t2 = now_ms();
         p->stage_ms[1] += t2 - t1;
         real Rin[9] = {p->pose[0], p->pose[1], p->pose[2], p->pose[4], p->pose[5], p->pose[6],
@@ -982,7 +1050,8 @@ int KFO(pipeline_frame)(pipe_t* p, const uint16_t* raw)
         t0 = now_ms();
     }
     if (status == 0)
-        p->touched = KFO(fuse)(p->F, p->W, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0]);
+        p->touched = p->fmt ? fuse_impl(p->Fq, p->Wq, 1, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0])
+                            : fuse_impl(p->F, p->W, 0, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0]);
     p->stage_ms[3] += now_ms() - t0;
     p->frame++;
     return status;
@@ -1003,3 +1072,21 @@ const real* KFO(pipeline_depth)(const pipe_t* p, int which, int level)
 }
 double KFO(pipeline_stage_ms)(const pipe_t* p, int stage) { return p->stage_ms[stage]; }
 int64_t KFO(pipeline_touched)(const pipe_t* p) { return p->touched; }
+#ifndef KFREF_F64
+/* Switches a fresh pipeline to the precision form's volume (before the first frame). */
+int kfo_pipeline_use_snorm16(pipe_t* p)
+{
+    if (p->frame != 0) return -1;
+    int64_t cnt = (int64_t)p->n * p->n * p->n;
+    p->Fq = (int16_t*)malloc(sizeof(int16_t) * cnt);
+    p->Wq = (uint8_t*)malloc(sizeof(uint8_t) * cnt);
+    if (!p->Fq || !p->Wq) return -1;
+    kfo_initialize_grid_q(p->Fq, p->Wq, p->n, 2.0f, 0.0f);
+    free(p->F); free(p->W);
+    p->F = NULL; p->W = NULL;
+    p->fmt = 1;
+    return 0;
+}
+const int16_t* kfo_pipeline_tsdf_q(const pipe_t* p) { return p->Fq; }
+const uint8_t* kfo_pipeline_weight_q(const pipe_t* p) { return p->Wq; }
+#endif

@@ -60,10 +60,17 @@ void    kfo_initialize_grid(float* F, float* W, int n, float f0, float w0);
 int64_t kfo_fuse(float* F, float* W, int n, float width, float mue,
                  const float* depth, int w, int h, const float inv_pose[16], const float K[9]);
 
+/* precision form: snorm16 F (q / 32767), uint8 observation counts (cap 128) */
+void    kfo_initialize_grid_q(int16_t* F, uint8_t* W, int n, float f0, float w0);
+int64_t kfo_fuse_q(int16_t* F, uint8_t* W, int n, float width, float mue,
+                   const float* depth, int w, int h, const float inv_pose[16], const float K[9]);
+
 /* ---- raycast (project/src/raycast.cu) ---- */
 void kfo_raycast(const float* F, int n, float width, float mue, const float pose[16], const float K_inv[9],
                  int w, int h, kfo_float4* vout, kfo_float4* nout);
 float kfo_sample_tsdf(const float* F, int n, float u, float v, float s);
+void kfo_raycast_q(const int16_t* F, int n, float width, float mue, const float pose[16], const float K_inv[9],
+                   int w, int h, kfo_float4* vout, kfo_float4* nout);
 
 /* ---- ICP (project/src/icp.cu, icp.cuh, icp.cpp, linear_least_squares.cpp) ---- */
 void kfo_construct_icp_residuals(const kfo_float4* vmap, const kfo_float4* tvmap, const kfo_float4* tnmap,
@@ -100,6 +107,9 @@ const kfo_float4* kfo_pipeline_map(const kfo_pipeline* p, int which, int level);
 const float* kfo_pipeline_depth(const kfo_pipeline* p, int which, int level);
 double kfo_pipeline_stage_ms(const kfo_pipeline* p, int stage);
 int64_t kfo_pipeline_touched(const kfo_pipeline* p);
+int    kfo_pipeline_use_snorm16(kfo_pipeline* p);
+const int16_t* kfo_pipeline_tsdf_q(const kfo_pipeline* p);
+const uint8_t* kfo_pipeline_weight_q(const kfo_pipeline* p);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,18 @@ def _declare(l):
         fn("pipeline_depth", P, P, I, I)
         fn("pipeline_stage_ms", D, P, I)
         fn("pipeline_touched", I64, P)
+    l.kfo_fuse_q.restype = I64
+    l.kfo_fuse_q.argtypes = [P, P, I, F, F, P, I, I, P, P]
+    l.kfo_initialize_grid_q.restype = None
+    l.kfo_initialize_grid_q.argtypes = [P, P, I, F, F]
+    l.kfo_raycast_q.restype = None
+    l.kfo_raycast_q.argtypes = [P, I, F, F, P, P, I, I, P, P]
+    l.kfo_pipeline_use_snorm16.restype = I
+    l.kfo_pipeline_use_snorm16.argtypes = [P]
+    l.kfo_pipeline_tsdf_q.restype = P
+    l.kfo_pipeline_tsdf_q.argtypes = [P]
+    l.kfo_pipeline_weight_q.restype = P
+    l.kfo_pipeline_weight_q.argtypes = [P]
     l.kfo_threads.restype = I
     l.kfo_threads.argtypes = []
     l.kfo_expf.restype = F
@@ -169,6 +181,30 @@ class Oracle:
         self.f("initialize_grid")(_p(F), _p(W), n, f0, w0)
         return F, W
 
+    def new_volume_q(self, n, f0=2.0, w0=0.0):
+        """Precision-form volume: snorm16 F, uint8 W (FP32 oracle only)."""
+        F = np.empty(n * n * n, np.int16)
+        W = np.empty(n * n * n, np.uint8)
+        self.l.kfo_initialize_grid_q(_p(F), _p(W), n, f0, w0)
+        return F, W
+
+    def fuse_q(self, F, W, n, width, mue, depth, inv_pose, K):
+        depth = np.ascontiguousarray(depth, np.float32)
+        h, w = depth.shape
+        return self.l.kfo_fuse_q(_p(F), _p(W), n, width, mue, _p(depth), w, h,
+                                 _p(np.ascontiguousarray(inv_pose, np.float32)), _p(np.ascontiguousarray(K, np.float32)))
+
+    def raycast_q(self, F, n, width, mue, pose, Kinv, w, h, vout=None, nout=None):
+        if vout is None:
+            vout = np.zeros((h, w, 4), np.float32)
+            vout[..., 3] = -2
+        if nout is None:
+            nout = np.zeros((h, w, 4), np.float32)
+            nout[..., 3] = -2
+        self.l.kfo_raycast_q(_p(F), n, width, mue, _p(np.ascontiguousarray(pose, np.float32)),
+                             _p(np.ascontiguousarray(Kinv, np.float32)), w, h, _p(vout), _p(nout))
+        return vout, nout
+
     def fuse(self, F, W, n, width, mue, depth, inv_pose, K):
         depth = self.arr(depth)
         h, w = depth.shape
@@ -251,7 +287,7 @@ class OraclePipeline:
     """The reference's per-frame loop (kinect_fusion.cpp:44-183) on the CPU."""
 
     def __init__(self, w, h, n, width, mue, fx, fy, cx, cy, depth_scale, iters, dist_th, angle_th,
-                 stop_angle=0.0, stop_dist=0.0, init_pose=None, dtype=np.float32):
+                 stop_angle=0.0, stop_dist=0.0, init_pose=None, dtype=np.float32, snorm16=False):
         self.o = Oracle(dtype)
         self.w, self.h, self.n = w, h, n
         it = np.array(iters, np.int32)
@@ -260,6 +296,11 @@ class OraclePipeline:
                                               dist_th, angle_th, stop_angle, stop_dist, _p(pose))
         if not self.h_:
             raise MemoryError("oracle pipeline allocation failed")
+        self.snorm16 = bool(snorm16)
+        if snorm16:
+            assert self.o.dt == np.float32
+            if self.o.l.kfo_pipeline_use_snorm16(self.h_) != 0:
+                raise MemoryError("oracle precision-form volume allocation failed")
 
     def close(self):
         if self.h_:
@@ -286,6 +327,13 @@ class OraclePipeline:
         ct = C.c_float if self.o.dt == np.float32 else C.c_double
         buf = (ct * count).from_address(ptr)
         return np.ctypeslib.as_array(buf).reshape(shape).copy()
+
+    def tsdf_q(self):
+        """(int16 F codes, uint8 W counts) of the precision form."""
+        n3 = self.n ** 3
+        f = np.ctypeslib.as_array((C.c_int16 * n3).from_address(self.o.l.kfo_pipeline_tsdf_q(self.h_))).copy()
+        w = np.ctypeslib.as_array((C.c_uint8 * n3).from_address(self.o.l.kfo_pipeline_weight_q(self.h_))).copy()
+        return f, w
 
     def tsdf(self):
         n3 = self.n ** 3

@@ -1,0 +1,298 @@
+// The reference's own C++ tests (test/icp_tests.cpp, test/cuda_grid_map_tests.cpp,
+// test/transformation_tests.cpp, test/example_test.cpp) restated against the C++
+// class surface include/kfusion/kfusion.hpp, which calls the HIP library through
+// the C ABI.  Runs on a GPU; invoked by tests/test_gpu_cpp_surface.py.
+//
+// Deviations from the reference tests: TestComputeNormal uses valid vertices
+// (the reference leaves the validity flag 0 and asserts only component [0]);
+// TestComputePose uses two frames of a seeded synthetic sequence instead of the
+// TUM freiburg1_xyz frames the reference loads from disk.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "kfusion/kfusion.hpp"
+
+extern "C" {
+int kfs_trajectory(uint64_t seed, int n_frames, double* poses);
+int kfs_spheres(uint64_t seed, double L, const double* poses, int n_frames, int n_spheres, double* spheres);
+void kfs_render(double L, const double* spheres, int n_sph, const double* poses, int first, int count, int w, int h,
+                double fx, double fy, double cx, double cy, uint64_t seed, double noise_k, double dropout,
+                uint16_t* out, double* true_depth);
+}
+
+using namespace kfusion;
+
+static int g_failed = 0, g_checks = 0;
+#define CHECK(cond)                                                                  \
+    do {                                                                             \
+        ++g_checks;                                                                  \
+        if (!(cond)) {                                                               \
+            ++g_failed;                                                              \
+            std::printf("  CHECK FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);    \
+        }                                                                            \
+    } while (0)
+#define CHECK_NEAR(a, b, tol) CHECK(std::fabs((double)(a) - (double)(b)) <= (tol))
+
+static const float pi = 3.14159265358979323846f;
+// icp_tests.cpp:33-42
+static const Mat3 intrinsics = {2.0f, 0.0f, 0.0f, 0.0f, 2.0f, 0.0f, 1.0f, 1.0f, 1.0f};
+static const Mat3 rotation_mat = {0.0f, -1.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+static const Mat3 no_rotation = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t n = 0;
+    explicit DeviceBuffer(size_t bytes) : n(bytes) { check(kf_malloc(&p, bytes), "kf_malloc"); }
+    ~DeviceBuffer() { kf_free(p); }
+    float* f() const { return static_cast<float*>(p); }
+};
+
+// one-pixel maps: a row for a given vertex (s), target vertex (d) and target normal (n)
+static std::array<float, 7> one_row(const Vec3& s, const Vec3& d, const Vec3& n)
+{
+    // a 2x2 map whose pixel (0,0) sees s; its right and lower neighbours make the
+    // live normal valid; identity poses and K = diag(1,1,1) with c = 0 map s onto (0, 0)
+    GridMap v(2, 2, MapFormat::Vector4), tv(2, 2, MapFormat::Vector4), tn(2, 2, MapFormat::Vector4);
+    float vh[16] = {s[0], s[1], s[2], -1, s[0] + 0.01f, s[1], s[2], -1, s[0], s[1] + 0.01f, s[2], -1, 0, 0, 0, -2};
+    float tvh[16] = {d[0], d[1], d[2], -1, 0, 0, 0, -2, 0, 0, 0, -2, 0, 0, 0, -2};
+    float tnh[16] = {n[0], n[1], n[2], -1, 0, 0, 0, -2, 0, 0, 0, -2, 0, 0, 0, -2};
+    v.upload(vh);
+    tv.upload(tvh);
+    tn.upload(tnh);
+    // principal point far away so every vertex projects to pixel (0, 0)
+    const Mat3 K = {1e-6f, 0, 0, 0, 1e-6f, 0, 0.5f, 0.5f, 1};
+    DeviceBuffer A(4 * 6 * 4), b(4 * 4);
+    kernel::constructIcpResiduals(v, tv, tn, no_rotation, {0, 0, 0}, no_rotation, {0, 0, 0}, K, 1e9f, pi, A.f(), b.f());
+    std::array<float, 24> Ah;
+    std::array<float, 4> bh;
+    check(kf_memcpy_d2h(Ah.data(), A.p, sizeof(Ah), nullptr), "d2h");
+    check(kf_memcpy_d2h(bh.data(), b.p, sizeof(bh), nullptr), "d2h");
+    return {Ah[0], Ah[1], Ah[2], Ah[3], Ah[4], Ah[5], bh[0]};
+}
+
+// ---------------------------------------------------------------- icp_tests.cpp
+static void TestComputeAndFillA()
+{
+    // icp_tests.cpp:100-114: vertex (2,3,1), normal (1,4,2) -> (-2, 3, -5, 1, 4, 2)
+    auto r = one_row({2, 3, 1}, {2, 3, 1}, {1, 4, 2});
+    const float expect[6] = {-2, 3, -5, 1, 4, 2};
+    for (int i = 0; i < 6; ++i) CHECK(r[i] == expect[i]);
+}
+
+static void TestComputeAndFillB()
+{
+    // icp_tests.cpp:116-127: s (2,3,1), n (1,4,2), d (1,2,0) -> -7
+    auto r = one_row({2, 3, 1}, {1, 2, 0}, {1, 4, 2});
+    CHECK(r[6] == -7.0f);
+}
+
+static void TestSolveLinearSystem()
+{
+    // icp_tests.cpp:129-157
+    float A[7][6] = {{1, 0, 0, 0, 0, 0}, {0, 2, 0, 0, 0, 0}, {0, 0, 3, 0, 0, 0}, {0, 0, 0, 4, 0, 0},
+                     {0, 0, 0, 0, 5, 0}, {0, 0, 0, 0, 0, 6}, {1, 1, 1, 1, 1, 1}};
+    float b[7] = {1, 2, 3, 4, 5, 6, 6};
+    DeviceBuffer Ad(sizeof(A)), bd(sizeof(b));
+    check(kf_memcpy_h2d(Ad.p, A, sizeof(A), nullptr), "h2d");
+    check(kf_memcpy_h2d(bd.p, b, sizeof(b), nullptr), "h2d");
+    LinearLeastSquares solver;
+    auto x = solver.solve(Ad.f(), bd.f(), 7);
+    for (int i = 0; i < 6; ++i) CHECK_NEAR(1.0, x[i], 0.0001);
+}
+
+static void TestConstructIcpResiduals()
+{
+    // icp_tests.cpp:159-231
+    float target_vertices[16] = {-1, -1, 3, -1, 1, -1, 3, -1, -1, 1, 3, -1, 1, 1, 3, -1};
+    float target_normals[16] = {0, 0, -1, -1, 0, 0, -1, -1, 0, 0, -1, -1, 0, 0, -1, -1};
+    float vertices[16] = {-1, -1, 4, -1, 1, -1, 4, -1, -1, 1, 5, -1, 1, 1, 4, -1};
+    GridMap tv(2, 2, MapFormat::Vector4), tn(2, 2, MapFormat::Vector4), v(2, 2, MapFormat::Vector4);
+    tv.upload(target_vertices);
+    tn.upload(target_normals);
+    v.upload(vertices);
+    DeviceBuffer A(4 * 6 * 4), b(4 * 4);
+    kernel::constructIcpResiduals(v, tv, tn, no_rotation, {0, 0, 0}, no_rotation, {0, 0, 0}, intrinsics, 1.5f, pi,
+                                  A.f(), b.f());
+    float Ah[4][6], bh[4];
+    check(kf_memcpy_d2h(Ah, A.p, sizeof(Ah), nullptr), "d2h");
+    check(kf_memcpy_d2h(bh, b.p, sizeof(bh), nullptr), "d2h");
+    const float true_a[4][6] = {{-1, 1, 0, 0, 0, -1}, {0}, {0}, {0}};
+    const float true_b[4] = {1, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j < 6; ++j) CHECK(Ah[i][j] == true_a[i][j]);
+        CHECK(bh[i] == true_b[i]);
+    }
+}
+
+static void TestComputePose()
+{
+    // icp_tests.cpp:233-326 on two frames of a synthetic sequence (seed 1):
+    // rotation within pi/90 and translation within 1 cm of the ground truth.
+    const int W = 640, H = 480, frames = 2;
+    std::vector<double> poses(frames * 16), sph(8 * 4);
+    CHECK(kfs_trajectory(1, frames, poses.data()) == 0);
+    CHECK(kfs_spheres(1, 3.0, poses.data(), frames, 8, sph.data()) >= 0);
+    std::vector<uint16_t> raw((size_t)frames * W * H);
+    kfs_render(3.0, sph.data(), 8, poses.data(), 0, frames, W, H, 525, 525, 319.5, 239.5, 1, 0.0012, 0.0,
+               raw.data(), nullptr);
+    IcpConfig icp_config{{8, 4, 2}, 640, 480, 0.5f, 6.0f * pi / 18.0f};
+    std::vector<std::unique_ptr<GridMapPyramid<GridMap>>> vp, np;
+    Sensor sensors[2] = {Sensor(525.0f), Sensor(525.0f)};
+    for (int f = 0; f < frames; ++f) {
+        DepthMap raw_map(W, H);
+        raw_map.update(raw.data() + (size_t)f * W * H);
+        GridMap meters(W, H, MapFormat::Depth32);
+        kernel::convertToDepthMeters(raw_map, meters, 1.0f / 1000.0f, false);
+        GridMapPyramid<GridMap> dp(W, H, 3, MapFormat::Depth32);
+        kernel::applyBilateralFilter(meters, dp[0]);
+        kernel::downSample(dp[0], dp[1]);
+        kernel::downSample(dp[1], dp[2]);
+        vp.push_back(std::make_unique<GridMapPyramid<GridMap>>(W, H, 3, MapFormat::Vector4));
+        np.push_back(std::make_unique<GridMapPyramid<GridMap>>(W, H, 3, MapFormat::Vector4));
+        Mat4 P;
+        for (int i = 0; i < 16; ++i) P[i] = (float)poses[f * 16 + i];
+        sensors[f].setPose(P);
+        for (int l = 0; l < 3; ++l) {
+            kernel::createVertexMap(dp[l], (*vp[f])[l], sensors[f].getInverseIntr(l));
+            kernel::createNormalMap((*vp[f])[l], (*np[f])[l]);
+        }
+    }
+    // the reference aligns frame 2 against frame 1's camera-space maps from the identity (icp_tests.cpp:304-313)
+    // true relative pose = inverse(P1) * P2
+    double P1[16], P2[16];
+    for (int i = 0; i < 16; ++i) { P1[i] = poses[i]; P2[i] = poses[16 + i]; }
+    double rel[3][4];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) {
+            double acc = 0;
+            for (int q = 0; q < 3; ++q) acc += P1[r * 4 + q] * (P2[c * 4 + q] - (c == 3 ? P1[12 + q] : 0.0));
+            rel[r][c] = acc;
+        }
+    Sensor s2(525.0f);
+    s2.setPose(identity4());
+    ICP icp(icp_config);
+    auto est = icp.computePose(*vp[1], *vp[0], *np[0], s2);
+    // angle between R_true v and R_est v for v = normalize(1,1,1) (icp_tests.cpp:315-320)
+    double v[3] = {1 / std::sqrt(3.0), 1 / std::sqrt(3.0), 1 / std::sqrt(3.0)}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) {
+            a[r] += rel[r][q] * v[q];
+            b[r] += est.rot_mat[q * 3 + r] * v[q];
+        }
+    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    double na = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    double angle = std::acos(std::fmin(1.0, dot / (na * nb)));
+    CHECK(angle <= pi / 90.0f);
+    for (int i = 0; i < 3; ++i) CHECK_NEAR(rel[i][3], est.transl_vec[i], 0.01);
+    auto t = icp.getExecutionTimes();
+    CHECK(t[0] > 0.0f && t[1] > 0.0f);
+}
+
+// ---------------------------------------------------------------- transformation_tests.cpp
+static void TestComputeNormal()
+{
+    float vertices[16] = {0, 0, 0, -1, 2, 0, 0, -1, 0, 4, 0, -1, 3, 3, 3, -1};
+    GridMap v(2, 2, MapFormat::Vector4), n(2, 2, MapFormat::Vector4);
+    v.upload(vertices);
+    kernel::createNormalMap(v, n);
+    float out[16];
+    n.download(out);
+    CHECK(out[0] == 0.0f && out[1] == 0.0f && out[2] == 1.0f && out[3] == -1.0f);
+}
+
+// ---------------------------------------------------------------- cuda_grid_map_tests.cpp
+static void TestCreate3LayerPyramid()
+{
+    GridMapPyramid<GridMap> p1(4, 4, 3, MapFormat::Vector4), p2(4, 4, 3, MapFormat::Vector4);
+    CHECK(p1[0].data() != p1[1].data());
+    CHECK(p1[0].data() != p2[0].data());
+    CHECK((p1[2].getGridDims() == std::array<unsigned int, 2>{1, 1}));
+    bool threw = false;
+    try {
+        GridMapPyramid<GridMap> bad(10, 8, 3, MapFormat::Vector4);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+static void TestUpdateDepthMap()
+{
+    float src[16], dst[16] = {0};
+    for (int i = 0; i < 16; ++i) src[i] = (float)i;
+    DepthMap d(4, 4, MapFormat::Depth32);
+    d.update(src);
+    d.download(dst);
+    for (int i = 0; i < 16; ++i) CHECK(src[i] == dst[i]);
+}
+
+// ---------------------------------------------------------------- example_test.cpp
+static void TestGlmToArray()
+{
+    Mat3 m = {0, 1, 0, -1, 42, 0, 0, 0, 1};
+    CHECK(m.data()[4] == 42.0f);
+}
+
+// ---------------------------------------------------------------- error behaviour
+static void TestDimensionMismatchThrows()
+{
+    GridMap a(8, 8, MapFormat::Depth32), b(8, 4, MapFormat::Depth32);
+    bool threw = false;
+    try {
+        kernel::applyBilateralFilter(a, b);
+    } catch (const std::runtime_error& e) {
+        threw = std::string(e.what()).find("not of same size") != std::string::npos;
+    }
+    CHECK(threw);
+}
+
+static void TestSaveLoadVoxelGrid()
+{
+    auto g = VoxelGrid::withMue(2.0f, 16, 0.1f);
+    auto fw = g.download();
+    CHECK(fw.size() == 16 * 16 * 16 * 2 && fw[0] == 2.0f && fw[1] == 0.0f);
+    const std::string path = "/tmp/kf_class_surface_grid.bin";
+    g.saveVoxelGrid(path);
+    auto g2 = VoxelGrid::withMue(2.0f, 16, 0.1f);
+    kernel::initializeGrid(g2, Voxel{0.5f, 3.0f});
+    g2.loadVoxelGrid(path);
+    CHECK(g2.download() == fw);
+    std::remove(path.c_str());
+}
+
+int main()
+{
+    check(kf_device_init(0), "kf_device_init");
+    const std::vector<std::pair<const char*, std::function<void()>>> tests = {
+        {"TestComputeAndFillA", TestComputeAndFillA},
+        {"TestComputeAndFillB", TestComputeAndFillB},
+        {"TestSolveLinearSystem", TestSolveLinearSystem},
+        {"TestConstructIcpResiduals", TestConstructIcpResiduals},
+        {"TestComputePose", TestComputePose},
+        {"TestComputeNormal", TestComputeNormal},
+        {"TestCreate3LayerPyramid", TestCreate3LayerPyramid},
+        {"TestUpdateDepthMap", TestUpdateDepthMap},
+        {"TestGlmToArray", TestGlmToArray},
+        {"TestDimensionMismatchThrows", TestDimensionMismatchThrows},
+        {"TestSaveLoadVoxelGrid", TestSaveLoadVoxelGrid},
+    };
+    for (auto& t : tests) {
+        const int before = g_failed;
+        std::printf("[ RUN  ] %s\n", t.first);
+        std::fflush(stdout);
+        try {
+            t.second();
+        } catch (const std::exception& e) {
+            ++g_failed;
+            std::printf("  EXCEPTION %s\n", e.what());
+        }
+        std::printf("[ %s ] %s\n", g_failed == before ? " OK " : "FAIL", t.first);
+        std::fflush(stdout);
+    }
+    std::printf("%d checks, %d failed\n", g_checks, g_failed);
+    return g_failed == 0 ? 0 : 1;
+}

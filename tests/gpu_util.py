@@ -24,8 +24,11 @@ def rt(pose16):
 
 
 def rot_err(Ra, Rb):
-    """Angle (rad) between two column-major rotations."""
+    """Angle (rad) between two column-major rotations, atan2 of the axis and cosine
+    parts of A^T B (arccos of the trace alone is ill-conditioned near identity)."""
     A = np.asarray(Ra, np.float64).reshape(3, 3).T
     B = np.asarray(Rb, np.float64).reshape(3, 3).T
-    c = (np.trace(A.T @ B) - 1.0) / 2.0
-    return float(np.arccos(np.clip(c, -1.0, 1.0)))
+    M = A.T @ B
+    w = 0.5 * np.array([M[2, 1] - M[1, 2], M[0, 2] - M[2, 0], M[1, 0] - M[0, 1]])
+    c = (np.trace(M) - 1.0) / 2.0
+    return float(np.arctan2(np.linalg.norm(w), c))

@@ -64,7 +64,7 @@ def test_raycast_bit_exact(kf, oracle32):
         kf.kernel.raycast(grid, sensor, vm, nm, lvl)
         rv, rn = oracle32.raycast(F, n, L, mue, sensor.get_pose(), sensor.get_inverse_intr(lvl), lw, lh)
         gv, gn = vm.download(), nm.download()
-        assert (rv[..., 3] == -1).mean() > 0.9
+        assert (rv[..., 3] == -1).mean() > 0.8
         np.testing.assert_array_equal(gv.view(np.uint32), rv.view(np.uint32))
         np.testing.assert_array_equal(gn.view(np.uint32), rn.view(np.uint32))
 
