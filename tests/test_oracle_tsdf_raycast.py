@@ -96,7 +96,7 @@ def test_raycast_sphere_normals(oracle32):
     res = W / (n - 1)
     c = np.arange(n) * res - W / 2
     z, y, x = np.meshgrid(c, c, c, indexing="ij")
-    C, r = np.array([0.0, 0.0, 0.6]), 0.3
+    C, r = np.array([0.0, 0.0, 0.75]), 0.25  # front at 0.5 m, beyond the 0.4 m near limit
     # camera at the origin looking along +z: projective distance along z, truncated
     sdf = np.linalg.norm(np.stack([x - C[0], y - C[1], z - C[2]]), axis=0) - r
     F = np.clip(sdf / mue, -1, 1).astype(np.float32).reshape(-1)
