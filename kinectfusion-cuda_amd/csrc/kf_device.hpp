@@ -162,11 +162,19 @@ __device__ __forceinline__ int order_key(float f)
 }
 __device__ __forceinline__ float order_value(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
 
-// snorm16 TSDF codes (precision form): f = q / 32767 with q in [-32767, 32767].
+// snorm16 TSDF codes (precision form): f = q / 32767 with q in [-32767, 32767];
+// the code snorm16 leaves unused, -32768, holds the unobserved value 2 of
+// tsdf.cu:7-18 (any f > 1 is stored as it), so samples next to unobserved
+// space interpolate with 2 exactly as the FP32 volume does.
 constexpr float kSnormScale = 32767.0f;
-__device__ __forceinline__ float snorm_decode(int16_t q) { return (float)q * (1.0f / kSnormScale); }
+constexpr int kSnormUnobserved = -32768;
+__device__ __forceinline__ float snorm_decode(int16_t q)
+{
+    return q == kSnormUnobserved ? 2.0f : (float)q * (1.0f / kSnormScale);
+}
 __device__ __forceinline__ int16_t snorm_encode(float f)
 {
+    if (f > 1.0f) return (int16_t)kSnormUnobserved;
     float s = f * kSnormScale;
     s = fminf(fmaxf(s, -kSnormScale), kSnormScale);
     return (int16_t)rintf(s);

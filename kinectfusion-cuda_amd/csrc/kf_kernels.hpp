@@ -24,6 +24,28 @@ int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, 
 int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s);
 
 // ---- raycast (raycast.hip)
+struct RaycastVolume {
+    const void* f;
+    const float* brick_min;  // per-brick lower bound of F (null: no skipping)
+    const int* super_key;    // per-super-brick lower bound of F, order_key form
+    int n, nb, ns, format;
+    float width, mue;
+};
+struct RaycastLevel {
+    float Kinv[9];
+    float4* vout;
+    float4* nout;
+    int w, h, tiles_x, tile0;  // tile0: first global tile of this level
+};
+// One (session, level) of a batched raycast; lives in device memory.
+struct RaycastJob {
+    RaycastVolume vol;
+    const float* pose;   // device: camera-to-world pose of the session (PoseState::pose)
+    const int* active;   // device flag, null = always active
+    RaycastLevel L;
+};
+RaycastVolume raycast_volume(const kf_volume* v);
+int launch_raycast_batch(const RaycastJob* jobs_dev, int njobs, int tiles, hipStream_t s);
 int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
                    float4* vout, float4* nout, int w, int h, hipStream_t s);
 // All pyramid levels in one launch (level l is w0 >> l by h0 >> l).

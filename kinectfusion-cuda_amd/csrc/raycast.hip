@@ -28,12 +28,6 @@ constexpr int kMaxMarchSteps = 1 << 20;
 // shorten the step.
 constexpr float kSkipBound = 0.991f;
 
-struct RaycastLevel {
-    float Kinv[9];
-    float4* vout;
-    float4* nout;
-    int w, h, tiles_x, tile0;  // tile0: first global tile of this level
-};
 
 // The fast skip test works on texel coordinates computed with a reciprocal
 // instead of the exact division (a few 1e-5 voxel apart): a sample counts as
@@ -41,12 +35,9 @@ struct RaycastLevel {
 // so its exact stencil is inside that region too.
 constexpr float kMargin = 0.01f;
 
+
 struct RaycastArgs {
-    const void* f;
-    const float* brick_min;  // per-brick lower bound of F (null: no skipping)
-    const int* super_key;    // per-super-brick lower bound of F, order_key form
-    int n, nb, ns, format;
-    float width, mue;
+    RaycastVolume vol;
     float pose[16];
     const PoseState* dyn;  // when set, the pose comes from dyn->pose
     RaycastLevel lv[3];
@@ -109,7 +100,7 @@ __device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float
 }
 
 // True when the sample is known to be >= 0.99 without reading the voxels.
-__device__ __forceinline__ bool skippable(const RaycastArgs& a, const Stencil& t)
+__device__ __forceinline__ bool skippable(const RaycastVolume& a, const Stencil& t)
 {
     if (!a.brick_min) return false;
     const int b0 = ((t.k0 >> 3) * a.nb + (t.j0 >> 3)) * a.nb + (t.i0 >> 3);
@@ -121,7 +112,7 @@ __device__ __forceinline__ bool skippable(const RaycastArgs& a, const Stencil& t
 // only if the exact stencil lies inside one super-brick (or one brick) whose
 // lower bound of F is >= kSkipBound.  The last super-brick's bound is cached
 // in (cs, cb), so a march through free space reads no memory at all.
-__device__ __forceinline__ bool fast_skip(const RaycastArgs& a, float xa, float ya, float za, int& cs, float& cb)
+__device__ __forceinline__ bool fast_skip(const RaycastVolume& a, float xa, float ya, float za, int& cs, float& cb)
 {
     const float hi = (float)a.n - 1.0f - kMargin;
     if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return false;
@@ -145,24 +136,10 @@ __device__ __forceinline__ bool fast_skip(const RaycastArgs& a, float xa, float 
     return false;
 }
 
+// raycast.cu:44-145 for pixel (i, j) of one level map.
 template <int FMT>
-__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
+__device__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i, int j)
 {
-    // XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
-    // contiguous band of tiles.
-    const int g = blockIdx.x;
-    const int per_xcd = (a.tiles + 7) / 8;
-    const int tile = (g % 8) * per_xcd + g / 8;
-    if (tile >= a.tiles) return;
-    int l = 0;
-    while (l + 1 < a.levels && tile >= a.lv[l + 1].tile0) ++l;
-    const RaycastLevel& L = a.lv[l];
-    const int lt = tile - L.tile0;
-    const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
-    const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-    if (i >= L.w || j >= L.h) return;
-
-    const float* pose = a.dyn ? a.dyn->pose : a.pose;
     float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     float M[9];
     m3m3(Rm, L.Kinv, M);
@@ -262,22 +239,84 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
     L.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
 }
 
+// XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
+// contiguous band of tiles.
+__device__ __forceinline__ int xcd_tile(int tiles)
+{
+    const int g = blockIdx.x;
+    const int per_xcd = (tiles + 7) / 8;
+    return (g % 8) * per_xcd + g / 8;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
+{
+    const int tile = xcd_tile(a.tiles);
+    if (tile >= a.tiles) return;
+    int l = 0;
+    while (l + 1 < a.levels && tile >= a.lv[l + 1].tile0) ++l;
+    const RaycastLevel& L = a.lv[l];
+    const int lt = tile - L.tile0;
+    const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
+    const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
+    if (i >= L.w || j >= L.h) return;
+    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j);
+}
+
+// Many sessions and levels in one launch: jobs[k] covers tiles
+// [jobs[k].L.tile0, jobs[k+1].L.tile0); a job whose session is inactive this
+// tick returns at once.
+__global__ __launch_bounds__(64) void raycast_batch_kernel(const RaycastJob* jobs, int njobs, int tiles)
+{
+    const int tile = xcd_tile(tiles);
+    if (tile >= tiles) return;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].L.tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const RaycastJob& J = jobs[lo];
+    if (J.active && !*J.active) return;
+    const int lt = tile - J.L.tile0;
+    const int i = (lt % J.L.tiles_x) * 8 + (threadIdx.x & 7);
+    const int j = (lt / J.L.tiles_x) * 8 + (threadIdx.x >> 3);
+    if (i >= J.L.w || j >= J.L.h) return;
+    if (J.vol.format == KF_VOXEL_F32) raycast_pixel<KF_VOXEL_F32>(J.vol, J.pose, J.L, i, j);
+    else raycast_pixel<KF_VOXEL_SNORM16_U8>(J.vol, J.pose, J.L, i, j);
+}
+
 }  // namespace
+
+RaycastVolume raycast_volume(const kf_volume* v)
+{
+    RaycastVolume r{};
+    r.f = v->f;
+    r.brick_min = v->brick_min;
+    r.super_key = v->super_key;
+    r.n = v->n;
+    r.nb = v->nb;
+    r.ns = v->ns;
+    r.format = v->format;
+    r.width = v->width;
+    r.mue = v->mue;
+    return r;
+}
+
+int launch_raycast_batch(const RaycastJob* jobs_dev, int njobs, int tiles, hipStream_t s)
+{
+    if (njobs <= 0 || tiles <= 0) return KF_OK;
+    raycast_batch_kernel<<<(unsigned)(((tiles + 7) / 8) * 8), 64, 0, s>>>(jobs_dev, njobs, tiles);
+    KF_LAUNCHED("raycast_batch_kernel");
+    return KF_OK;
+}
 
 int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState* dyn_state, int levels,
                           const float (*inv_k)[9], float4* const* vout, float4* const* nout, int w0, int h0,
                           hipStream_t s)
 {
     RaycastArgs a{};
-    a.f = v->f;
-    a.brick_min = v->brick_min;
-    a.super_key = v->super_key;
-    a.n = v->n;
-    a.nb = v->nb;
-    a.ns = v->ns;
-    a.format = v->format;
-    a.width = v->width;
-    a.mue = v->mue;
+    a.vol = raycast_volume(v);
     if (pose) memcpy(a.pose, pose, sizeof(a.pose));
     a.dyn = dyn_state;
     a.levels = levels;

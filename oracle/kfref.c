@@ -353,12 +353,20 @@ void KFO(initialize_grid)(real* F, real* W, int n, real f0, real w0)
  * Pinned by: the fronto-parallel wall closed form, repeat-invariance of F and
  * W = k/D after k identical frames (tests/test_oracle_tsdf.py). */
 /* Precision form (snorm16 F, uint8 W): F is stored as q = rint(clamp(f, -1, 1) * 32767)
- * and read as q / 32767; W counts observations: F = (F*W + f) / (W + 1),
- * W = min(W + 1, 128).  Same projection and tests as the FP32 form. */
+ * and read as q / 32767; the one code snorm16 leaves unused, -32768, holds the
+ * unobserved value 2 of tsdf.cu:7-18 (any f > 1 is stored as it), so trilinear
+ * samples next to unobserved space mix in 2 exactly as the FP32 volume does.
+ * W counts observations: F = (F*W + f) / (W + 1), W = min(W + 1, 128).  Same
+ * projection and tests as the FP32 form. */
 #define KF_SNORM R(32767.0)
-static inline real snorm_decode(int16_t q) { return (real)q * (R(1.0) / KF_SNORM); }
+#define KF_SNORM_UNOBSERVED (-32768)
+static inline real snorm_decode(int16_t q)
+{
+    return q == KF_SNORM_UNOBSERVED ? R(2.0) : (real)q * (R(1.0) / KF_SNORM);
+}
 static inline int16_t snorm_encode(real f)
 {
+    if (f > R(1.0)) return (int16_t)KF_SNORM_UNOBSERVED;
     real s = f * KF_SNORM;
     s = FMIN(FMAX(s, -KF_SNORM), KF_SNORM);
 #ifdef KFREF_F64

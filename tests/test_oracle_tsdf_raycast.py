@@ -66,7 +66,19 @@ def test_fuse_precision_form(oracle32):
     seen = Wr > 0
     assert (Wt[seen] == 3).all() and (Wt[~seen] == 0).all()
     np.testing.assert_array_less(np.abs(F[seen] / 32767.0 - Fr[seen]), 2.0 / 32767.0)
-    assert (F[~seen] == 32767).all()  # initial f = 2 clamps to +1
+    assert (F[~seen] == -32768).all()  # the unobserved code: initial f = 2
+
+
+def test_precision_form_unobserved_reads_as_two(oracle32):
+    """The unobserved code decodes to 2, so a raycast of the precision-form volume equals
+    the raycast of the FP32 volume whose F is the decoded codes (observation rims included)."""
+    n, W, mue, D = 48, 2.0, 0.1, 0.8
+    Fq, Wq, _, K, Ki = wall_case(oracle32, n=n, W=W, mue=mue, D=D, frames=2, fmt="q")
+    Fd = np.where(Fq == -32768, np.float32(2.0), Fq.astype(np.float32) * np.float32(1.0 / 32767.0))
+    vq, nq = oracle32.raycast_q(Fq, n, W, mue, I4, Ki, 64, 48)
+    vf, nf = oracle32.raycast(Fd.astype(np.float32), n, W, mue, I4, Ki, 64, 48)
+    np.testing.assert_array_equal(vq, vf)
+    np.testing.assert_array_equal(nq, nf)
 
 
 def test_raycast_wall_quirk_one_small_step(oracle32):
