@@ -126,7 +126,9 @@ int kf_volume_bytes(const kf_volume* v, size_t* bytes);
 /* Total number of voxels updated by every fuse into this volume so far. */
 int kf_volume_touched_total(const kf_volume* v, int64_t* count);
 /* kernel::initializeGrid (tsdf.cuh:15, tsdf.cu:79-90); value (f, w).  For the
- * snorm16 format f is clamped to [-1, 1] and w rounded to an integer count. */
+ * snorm16 format f below -1 clamps to -1, f above 1 is stored as the
+ * unobserved code (read back as 2, the reference's initial value), and w is
+ * rounded to an integer count. */
 int kf_initialize_grid(kf_volume* v, float f, float w, kf_stream_t stream, float* elapsed_ms);
 /* kernel::fuse (tsdf.cuh:16, tsdf.cu:92-103): projective TSDF update from an
  * unfiltered metric depth map seen by the sensor.  The reference hard-codes a
@@ -234,6 +236,127 @@ int kf_pipeline_set_timing(kf_pipeline* p, int enabled);
 int kf_pipeline_stage_times(kf_pipeline* p, float total_ms[KF_STAGE_COUNT], int64_t* frames);
 /* Average duration of the fuse launches timed since timing was enabled (ms). */
 int kf_pipeline_fuse_kernel_ms(kf_pipeline* p, float* avg_ms, int64_t* launches);
+
+/* ---------------------------------------------------------------- batch ---
+ * Many independent KinectFusion sessions (each its own volume, resolution,
+ * intrinsics and pose) advanced together: one tick runs one frame of every
+ * listed session with one launch per stage across all of them (preprocess,
+ * raycast L0..L2, each ICP iteration, fuse).  The reference runs one session
+ * per process (kinect_fusion.cpp:44-73); a session here computes exactly what
+ * kf_pipeline computes for the same frames (same kernels, same summation
+ * order), whatever else shares its ticks.  No reference counterpart: this
+ * is the multi-session form of kf_pipeline_frame. */
+typedef struct kf_session_config {
+    int frame_width, frame_height;  /* divisible by 4 (3-level pyramid) */
+    float depth_scale;
+    float fx, fy, cx, cy;
+    int volume_n;                   /* multiple of 8 */
+    float volume_size;
+    float mue;
+    int voxel_format;
+    float initial_pose[16];
+} kf_session_config;
+/* Launch shapes (workgroups per persistent launch); every shape gives the same
+ * results.  Zero fields take the defaults. */
+typedef struct kf_batch_shape {
+    unsigned preprocess_blocks, raycast_blocks, icp_blocks, cull_blocks, fuse_blocks;
+} kf_batch_shape;
+typedef struct kf_batch_config {
+    int max_sessions;
+    kf_icp_config icp;        /* iterations, thresholds and summation shared by every session (width/height unused) */
+    int use_graph;            /* 1: each tick is one hipGraph launch (captured once per table slot) */
+    int autotune;             /* 1: the first ticks time candidate launch shapes per stage and keep the fastest */
+    kf_batch_shape shape;
+} kf_batch_config;
+typedef struct kf_batch kf_batch;
+int kf_batch_create(const kf_batch_config* config, kf_stream_t stream, kf_batch** out);
+int kf_batch_destroy(kf_batch* b);
+/* HBM a session holds while admitted (volume + maps + ICP state), in bytes. */
+int kf_session_bytes(const kf_session_config* config, size_t* bytes);
+int kf_batch_add_session(kf_batch* b, const kf_session_config* config, int* session);
+int kf_batch_remove_session(kf_batch* b, int session);
+/* Enqueues one tick: frame raw[k] (device uint16, the session's size) for
+ * session sessions[k], k < n, each session at most once.  A session's first
+ * frame is fused at its initial pose; later frames are tracked then fused. */
+int kf_batch_tick(kf_batch* b, int n, const int* sessions, const uint16_t* const* raw);
+/* Waits for every enqueued tick. */
+int kf_batch_sync(kf_batch* b);
+int kf_batch_pose(kf_batch* b, int session, float pose[16]);
+/* First ICP failure of the session since the last call (KF_OK if none); clears it. */
+int kf_batch_session_status(kf_batch* b, int session, int* status);
+int kf_batch_volume(kf_batch* b, int session, kf_volume** v);
+/* which as kf_pipeline_map: 0 metres, 1 filtered L0, 2 vertex, 4 predicted vertex, 5 predicted normal. */
+int kf_batch_map(kf_batch* b, int session, int which, int level, void** dev_ptr, int* w, int* h);
+int kf_batch_shape_get(kf_batch* b, kf_batch_shape* shape, int* tuned);
+/* GPU time of the ticks since the last call (events around each tick). */
+int kf_batch_tick_times(kf_batch* b, double* total_ms, int64_t* ticks, float* max_ms);
+int kf_batch_touched_total(kf_batch* b, unsigned long long* voxels);
+
+/* -------------------------------------------------------------- serving ---
+ * A request loop over a kf_batch: sessions are admitted against an HBM budget
+ * (queued FIFO until slots and memory free up), every admitted volume stays
+ * resident, frames are queued per session as they arrive and each tick takes
+ * the oldest waiting frame of every session (continuous batching, at most two
+ * ticks in flight).  Per-frame latency = completion seen by the loop - arrival.
+ * No reference counterpart (the reference serves one camera per process). */
+typedef struct kf_server_config {
+    kf_batch_config batch;       /* max_sessions = resident session slots */
+    size_t hbm_budget_bytes;     /* admission budget over kf_session_bytes of the resident sessions */
+    int queue_depth;             /* frames buffered per session (pinned host memory); 0 = 8 */
+} kf_server_config;
+typedef struct kf_server_stats {
+    int64_t frames, ticks, admitted, queued_admissions, failed_frames;
+    double wall_s, frames_per_s, mean_frames_per_tick;
+    double latency_p50_ms, latency_p99_ms, latency_max_ms, latency_mean_ms;
+    size_t peak_resident_bytes;
+} kf_server_stats;
+typedef struct kf_server kf_server;
+int kf_server_create(const kf_server_config* config, kf_server** out);
+int kf_server_destroy(kf_server* s);
+/* Seconds on the loop's monotonic clock (arrival stamps use it). */
+double kf_server_now(void);
+/* Requests a session; *ticket names it from now on.  Admitted at once when a
+ * slot and the budget allow, else when earlier sessions close. */
+int kf_server_open(kf_server* s, const kf_session_config* config, int* ticket);
+/* The session ends after its queued frames; its slot and memory are freed. */
+int kf_server_close(kf_server* s, int ticket);
+/* Copies one host frame (uint16, the session's size) into the session's queue.
+ * KF_ERR_BUDGET when the queue is full. */
+int kf_server_submit(kf_server* s, int ticket, const uint16_t* frame, double arrival_s);
+/* One turn of the loop: retires finished ticks, admits waiting sessions,
+ * launches a tick when frames wait and fewer than two ticks are in flight.
+ * *launched (optional) = 1 when a tick was launched. */
+int kf_server_poll(kf_server* s, int* launched);
+/* Runs the loop until every queued frame is done and every closed session released. */
+int kf_server_drain(kf_server* s);
+int kf_server_stats_get(kf_server* s, kf_server_stats* out);
+/* Latencies (ms) of every finished frame, in completion order. */
+int kf_server_latencies(kf_server* s, double* out, size_t cap, size_t* count);
+/* 1 admitted, 0 waiting for admission, -1 closed and released. */
+int kf_server_state(kf_server* s, int ticket, int* state);
+int kf_server_pose(kf_server* s, int ticket, float pose[16]);
+int kf_server_volume(kf_server* s, int ticket, kf_volume** v);
+
+/* A Poisson load: session k opens at opens_s[k] (seconds after the start) and
+ * sends frames_per_session[k] frames with exponential inter-arrival times of
+ * mean 1/rate_hz, each scaled by a uniform factor in [1 - jitter, 1 + jitter]
+ * (seeded).  Frame f of session k is bank[k][p(f)] where p ping-pongs over the
+ * bank_len[k] frames (0, 1, ..., n-1, n-2, ..., 1, 0, 1, ...) so the camera
+ * path stays continuous.  The session closes after its last frame. */
+typedef struct kf_poisson_load {
+    int sessions;
+    const kf_session_config* configs;
+    const uint16_t* const* banks;
+    const int* bank_len;
+    const int* frames_per_session;
+    const double* opens_s;
+    double rate_hz, jitter;
+    uint64_t seed;
+    int keep_open;               /* 1: sessions stay resident after their last frame (close them yourself) */
+} kf_poisson_load;
+/* Drives the loop with the load in real time until every frame is done.
+ * tickets (optional, sessions entries) receive the sessions' tickets. */
+int kf_server_run_load(kf_server* s, const kf_poisson_load* load, int* tickets, kf_server_stats* out);
 
 #ifdef __cplusplus
 }

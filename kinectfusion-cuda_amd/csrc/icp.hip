@@ -20,6 +20,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "kf_device.hpp"
@@ -35,18 +36,6 @@ constexpr int kTerms = 27;
 // partial layout (component-major): [0,27) sums, [27,54) compensations, 54 count
 constexpr int kCountSlot = 54;
 
-struct IcpArgs {
-    const float4* V;
-    const float4* TV;
-    const float4* TN;
-    int w, h;
-    float K[9];
-    float dist_thresh, cos_thresh;
-    PoseState* st;  // pose source (R, t, Rp, tp) and early-stop flags
-    int level;
-    float* partials;
-    int stride;     // blocks of this launch (partial row stride)
-};
 
 // icp.cu:7-82 split in two: what a pixel needs that does not depend on the
 // pose (its vertex and its live normal, device_helper.cuh:81-110) and the
@@ -145,7 +134,7 @@ __device__ __forceinline__ void accumulate(float* s, float* c, const float* row,
 
 // Reduces the lane values of a workgroup and writes one partial row.
 template <bool COMP>
-__device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, float* partials, int stride)
+__device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, float* partials, int stride, int row)
 {
     __shared__ float red[kThreads / 64][2 * kTerms + 1];
 #pragma unroll
@@ -179,12 +168,12 @@ __device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, 
             if (COMP) comp_merge(ss, cc, red[wv][q], red[wv][kTerms + q]);
             else ss = ss + red[wv][q];
         }
-        partials[(int64_t)q * stride + blockIdx.x] = ss;
-        partials[(int64_t)(kTerms + q) * stride + blockIdx.x] = cc;
+        partials[(int64_t)q * stride + row] = ss;
+        partials[(int64_t)(kTerms + q) * stride + row] = cc;
     } else if (threadIdx.x == kTerms) {
         float n = 0.0f;
         for (int wv = 0; wv < kThreads / 64; ++wv) n = n + red[wv][2 * kTerms];
-        partials[(int64_t)kCountSlot * stride + blockIdx.x] = n;
+        partials[(int64_t)kCountSlot * stride + row] = n;
     }
 }
 
@@ -193,12 +182,12 @@ struct LanePixels {
     PixelIn in[kPixPerThread];
     bool live[kPixPerThread];
 };
-__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp)
+__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp, int block)
 {
     const int64_t npx = (int64_t)a.w * a.h;
 #pragma unroll
     for (int k = 0; k < kPixPerThread; ++k) {
-        int64_t px = ((int64_t)blockIdx.x * kPixPerThread + k) * kThreads + threadIdx.x;
+        int64_t px = ((int64_t)block * kPixPerThread + k) * kThreads + threadIdx.x;
         lp.live[k] = px < npx;
         int u = lp.live[k] ? (int)(px % a.w) : 0, v = lp.live[k] ? (int)(px / a.w) : 0;
         lp.in[k] = icp_pixel_in(a, u, v);
@@ -206,9 +195,10 @@ __device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& l
     }
 }
 
-// pose: R (9), t (3), Rp (9), tp (3) of the iteration, in LDS
+// pose: R (9), t (3), Rp (9), tp (3) of the iteration, in LDS.  Writes partial
+// row `block` of a.partials.
 template <bool COMP>
-__device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const float* pose)
+__device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const float* pose, int block)
 {
     const float* Rc = pose;
     const float* tc = pose + 9;
@@ -228,7 +218,7 @@ __device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const f
             ++cnt;
         }
     }
-    block_reduce_store<COMP>(s, c, cnt, a.partials, a.stride);
+    block_reduce_store<COMP>(s, c, cnt, a.partials, a.stride, block);
 }
 
 template <bool COMP>
@@ -237,13 +227,13 @@ __device__ void reduce_body(const IcpArgs& a)
     PoseState* st = a.st;
     __shared__ float pose[24];
     LanePixels lp;
-    load_lane_pixels(a, lp);
+    load_lane_pixels(a, lp, blockIdx.x);
     if (threadIdx.x < 9) pose[threadIdx.x] = st->R[threadIdx.x];
     else if (threadIdx.x < 12) pose[threadIdx.x] = st->t[threadIdx.x - 9];
     else if (threadIdx.x < 21) pose[threadIdx.x] = st->Rp[threadIdx.x - 12];
     else if (threadIdx.x < 24) pose[threadIdx.x] = st->tp[threadIdx.x - 21];
     __syncthreads();
-    associate_reduce<COMP>(a, lp, pose);
+    associate_reduce<COMP>(a, lp, pose, blockIdx.x);
 }
 
 template <bool COMP>
@@ -276,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void ab_reduce_kernel(const float* A, con
         for (int p = 0; p < 6; ++p) nz = nz || row[p] != 0.0f;
         cnt += nz ? 1 : 0;
     }
-    block_reduce_store<COMP>(s, c, cnt, partials, stride);
+    block_reduce_store<COMP>(s, c, cnt, partials, stride, blockIdx.x);
 }
 
 // icp.cu:7-82, writing the rows (kernel::constructIcpResiduals).
@@ -490,22 +480,9 @@ __global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, c
 // ---- the device-resident coarse-to-fine loop: one launch per iteration.
 // Launch k first finishes iteration k-1 (every workgroup sums launch k-1's
 // partial rows in the same fixed order and solves, so all of them hold the same
-// pose), then associates and reduces iteration k.  The carried state and the
-// partial rows are double-buffered by launch parity, so no workgroup reads what
-// another workgroup of the same launch writes.
-}  // namespace
-struct IcpCarry {
-    float R[9];
-    float t[3];
-    int status;
-    int level_done[4];
-    int iterations;
-    int pending;         // the writing launch left partial rows that still need their solve
-    int pending_level;
-    int pending_blocks;  // partial-row count of that launch
-    int pad[2];
-};
-namespace {
+// pose), then associates and reduces iteration k.  The carried state (IcpCarry)
+// and the partial rows are double-buffered by launch parity, so no workgroup
+// reads what another workgroup of the same launch writes.
 
 // Applies the pending solve of `c` (thread 0; tot/count already summed).
 __device__ void apply_pending(IcpCarry& c, const double* tot, double count, PoseState* st, bool record,
@@ -523,6 +500,34 @@ __device__ void apply_pending(IcpCarry& c, const double* tot, double count, Pose
     }
 }
 
+// Phase B of a launch for one session: finish the previous iteration from
+// (cin, pin) and put the pose of this iteration in LDS.  `first` = this
+// workgroup holds the session's partial row 0: it records the solve and writes
+// the carried state for the next launch.  Returns (via run) whether the
+// session iterates at `level` in this launch.  Ends with a barrier.
+template <bool COMP>
+__device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* cout, const float* pin, int level,
+                                int blocks, bool first, float stop_angle, float stop_dist, float* pose, int* run,
+                                double* tot, double* count)
+{
+    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
+    if (threadIdx.x == 0) {
+        IcpCarry c = *cin;
+        apply_pending(c, tot, *count, st, first, stop_angle, stop_dist);
+        const int go = (c.status == 0 && !c.level_done[level]) ? 1 : 0;
+        c.pending = go;
+        c.pending_level = level;
+        c.pending_blocks = blocks;
+        for (int q = 0; q < 9; ++q) pose[q] = c.R[q];
+        for (int q = 0; q < 3; ++q) pose[9 + q] = c.t[q];
+        for (int q = 0; q < 9; ++q) pose[12 + q] = st->Rp[q];
+        for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
+        *run = go;
+        if (first) *cout = c;
+    }
+    __syncthreads();
+}
+
 template <bool COMP>
 __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const IcpCarry* cin, IcpCarry* cout,
                                                             const float* pin, float stop_angle, float stop_dist)
@@ -531,30 +536,15 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
     __shared__ int run;
     __shared__ double tot[kTerms];
     __shared__ double count;
-    PoseState* st = a.st;
     // phase A: pose-independent loads of this lane's pixels
     LanePixels lp;
-    load_lane_pixels(a, lp);
+    load_lane_pixels(a, lp, blockIdx.x);
     // phase B: finish the previous iteration
-    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, &count);
-    if (threadIdx.x == 0) {
-        IcpCarry c = *cin;
-        apply_pending(c, tot, count, st, blockIdx.x == 0, stop_angle, stop_dist);
-        const int go = (c.status == 0 && !c.level_done[a.level]) ? 1 : 0;
-        c.pending = go;
-        c.pending_level = a.level;
-        c.pending_blocks = (int)gridDim.x;
-        for (int q = 0; q < 9; ++q) pose[q] = c.R[q];
-        for (int q = 0; q < 3; ++q) pose[9 + q] = c.t[q];
-        for (int q = 0; q < 9; ++q) pose[12 + q] = st->Rp[q];
-        for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
-        run = go;
-        if (blockIdx.x == 0) *cout = c;
-    }
-    __syncthreads();
+    begin_iteration<COMP>(a.st, cin, cout, pin, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle, stop_dist,
+                          pose, &run, tot, &count);
     if (!run) return;
     // phase C: association + reduction of this iteration
-    associate_reduce<COMP>(a, lp, pose);
+    associate_reduce<COMP>(a, lp, pose, blockIdx.x);
 }
 
 // Starts a frame's ICP: the working and previous poses are the sensor pose
@@ -562,11 +552,8 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
 struct Pose16 {
     float m[16];
 };
-__global__ void icp_begin_kernel(PoseState* st, IcpCarry* c0, Pose16 pose, int from_value)
+__device__ void begin_state(PoseState* st, IcpCarry* c0)
 {
-    if (threadIdx.x != 0) return;
-    if (from_value)
-        for (int q = 0; q < 16; ++q) st->pose[q] = pose.m[q];
     const float* P = st->pose;
     float Rm[9] = {P[0], P[1], P[2], P[4], P[5], P[6], P[8], P[9], P[10]};
     IcpCarry c;
@@ -584,20 +571,25 @@ __global__ void icp_begin_kernel(PoseState* st, IcpCarry* c0, Pose16 pose, int f
     c.pad[0] = c.pad[1] = 0;
     *c0 = c;
 }
+__global__ void icp_begin_kernel(PoseState* st, IcpCarry* c0, Pose16 pose, int from_value)
+{
+    if (threadIdx.x != 0) return;
+    if (from_value)
+        for (int q = 0; q < 16; ++q) st->pose[q] = pose.m[q];
+    begin_state(st, c0);
+}
 
 // Finishes the last iteration and commits the estimate as the sensor pose
 // (Sensor::setPose, kinect_fusion.cpp:146-161, sensor.h:30-34); on failure the
 // previous pose is kept and the frame is not fused.
 template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, const IcpCarry* cin, const float* pin,
-                                                              float stop_angle, float stop_dist)
+__device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin, float stop_angle, float stop_dist,
+                            double* tot, double* count)
 {
-    __shared__ double tot[kTerms];
-    __shared__ double count;
-    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, &count);
+    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
     if (threadIdx.x != 0) return;
     IcpCarry c = *cin;
-    apply_pending(c, tot, count, st, true, stop_angle, stop_dist);
+    apply_pending(c, tot, *count, st, true, stop_angle, stop_dist);
     for (int q = 0; q < 9; ++q) st->R[q] = c.R[q];
     for (int q = 0; q < 3; ++q) st->t[q] = c.t[q];
     for (int l = 0; l < 4; ++l) st->level_done[l] = c.level_done[l];
@@ -624,6 +616,85 @@ __global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, con
         I[12 + r] = (float)(-sum);
     }
     I[15] = 1.0f;
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, const IcpCarry* cin, const float* pin,
+                                                              float stop_angle, float stop_dist)
+{
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    commit_body<COMP>(st, cin, pin, stop_angle, stop_dist, tot, &count);
+}
+
+// ---- the same loop for many sessions per launch (batch ticks).  The jobs of
+// a level are sorted by block0; workgroup ranks own contiguous block ranges
+// (XCD-contiguous), and a workgroup solves a session's previous iteration the
+// first time it meets one of that session's blocks.  Per-session results do
+// not depend on the mix: partial rows are indexed by the session's own block
+// number and summed in that order.
+__device__ __forceinline__ const float* job_partials(const IcpJob& J, int k)
+{
+    return J.partials + (size_t)(k & 1) * kPartialStride * (size_t)J.max_blocks;
+}
+__device__ __forceinline__ float* job_partials_w(const IcpJob& J, int k)
+{
+    return J.partials + (size_t)(k & 1) * kPartialStride * (size_t)J.max_blocks;
+}
+
+__global__ void icp_batch_begin_kernel(const BatchHeader* hd, const IcpJob* jobs)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < hd->n_icp) begin_state(jobs[q].a.st, jobs[q].carry);
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_batch_step_kernel(const BatchHeader* hd, const IcpJob* jobs, int level,
+                                                                  int k, float stop_angle, float stop_dist)
+{
+    __shared__ float pose[24];
+    __shared__ int run;
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    const int njobs = hd->n_icp, total = hd->icp_blocks[level];
+    const int G = (int)gridDim.x;
+    const int rank = xcd_rank((int)blockIdx.x, G);
+    const int chunk = (total + G - 1) / G;
+    int b = rank * chunk;
+    const int end = min(total, b + chunk);
+    if (b >= end) return;
+    int j = job_of(jobs, njobs, b, [](const IcpJob& J) { return J.block0; });
+    int cur = -1;
+    for (; b < end; ++b) {
+        while (j + 1 < njobs && jobs[j + 1].block0 <= b) ++j;
+        const IcpJob& J = jobs[j];
+        const int lb = b - J.block0;
+        LanePixels lp;
+        load_lane_pixels(J.a, lp, lb);
+        __syncthreads();  // the previous block's use of pose / the reduction LDS is over
+        if (j != cur) {
+            cur = j;
+            begin_iteration<COMP>(J.a.st, J.carry + (k & 1), J.carry + ((k + 1) & 1), job_partials(J, k + 1), level,
+                                  J.a.stride, lb == 0, stop_angle, stop_dist, pose, &run, tot, &count);
+        }
+        if (!run) continue;
+        IcpArgs a = J.a;
+        a.partials = job_partials_w(J, k);
+        associate_reduce<COMP>(a, lp, pose, lb);
+    }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_batch_commit_kernel(const BatchHeader* hd, const IcpJob* jobs, int k,
+                                                                    float stop_angle, float stop_dist)
+{
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    for (int q = blockIdx.x; q < hd->n_icp; q += gridDim.x) {
+        const IcpJob& J = jobs[q];
+        commit_body<COMP>(J.a.st, J.carry + (k & 1), job_partials(J, k + 1), stop_angle, stop_dist, tot, &count);
+        __syncthreads();
+    }
 }
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads * kPixPerThread - 1) / (kThreads * kPixPerThread)); }
@@ -693,6 +764,33 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
 int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s)
 {
     return enqueue_steps(icp, levels, n_levels, s, nullptr, nullptr);
+}
+
+int launch_icp_batch(const BatchHeader* hd, const IcpJob* jobs, const int* iters, int levels, int max_sessions,
+                     int compensated, float stop_angle, float stop_dist, unsigned step_blocks, hipStream_t s)
+{
+    if (max_sessions <= 0) return KF_OK;
+    icp_batch_begin_kernel<<<(unsigned)((max_sessions + 63) / 64), 64, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("icp_batch_begin_kernel");
+    const IcpJob* level_jobs[3] = {jobs, jobs + max_sessions, jobs + 2 * max_sessions};
+    int k = 0;
+    for (int l = levels - 1; l >= 0; --l)
+        for (int it = 0; it < iters[l]; ++it, ++k) {
+            if (compensated)
+                icp_batch_step_kernel<true><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l, k, stop_angle,
+                                                                             stop_dist);
+            else
+                icp_batch_step_kernel<false><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l, k, stop_angle,
+                                                                              stop_dist);
+            KF_LAUNCHED("icp_batch_step_kernel");
+        }
+    const unsigned cb = (unsigned)std::min(max_sessions, 1024);
+    if (compensated)
+        icp_batch_commit_kernel<true><<<cb, kThreads, 0, s>>>(hd, jobs, k, stop_angle, stop_dist);
+    else
+        icp_batch_commit_kernel<false><<<cb, kThreads, 0, s>>>(hd, jobs, k, stop_angle, stop_dist);
+    KF_LAUNCHED("icp_batch_commit_kernel");
+    return KF_OK;
 }
 
 }  // namespace kf

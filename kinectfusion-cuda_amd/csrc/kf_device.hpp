@@ -180,4 +180,25 @@ __device__ __forceinline__ int16_t snorm_encode(float f)
     return (int16_t)rintf(s);
 }
 
+// Rank of workgroup g among G such that the ranks on one XCD (workgroups are
+// dealt to the 8 XCDs round-robin) are contiguous; identity unless G % 8 == 0.
+__device__ __forceinline__ int xcd_rank(int g, int G)
+{
+    return (G & 7) ? g : (g & 7) * (G >> 3) + (g >> 3);
+}
+
+// Index of the job whose range [key(job), key(job + 1)) holds `item`; the
+// jobs are sorted by key and key(jobs[0]) == 0.
+template <class J, class K>
+__device__ __forceinline__ int job_of(const J* jobs, int njobs, long long item, K key)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((long long)key(jobs[mid]) <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace kf

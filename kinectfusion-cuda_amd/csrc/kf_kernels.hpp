@@ -16,7 +16,63 @@ int launch_preprocess(const uint16_t* raw, int w, int h, float scale, int unmirr
                       const float (*inv_k)[9], float* depth_m, float* const* dpyr, float4* const* vpyr,
                       float4* const* npyr, hipStream_t s);
 
+// ---- batch ticks (batch.hip): one table per tick in device memory, a header
+// followed by the job arrays of every stage.  Counts are per tick; the fuse
+// brick list, its counter and the touched slots belong to the batch.
+struct BatchHeader {
+    int n_pre, pre_tiles, pyr_tiles;  // preprocess jobs, L0 16x16 tiles, L2 16x16 tiles
+    int n_ray, ray_tiles;             // raycast jobs (session x level), 8x8 tiles
+    int n_icp, icp_blocks[3];         // ICP sessions, blocks per level
+    int n_fuse;                       // fuse jobs
+    unsigned fuse_bricks;             // bricks over all fuse jobs
+    unsigned* fuse_count;
+    unsigned* fuse_list;
+    unsigned long long* touched;
+};
+
+// Preprocess of one session: raw codes -> metres (fuse input), bilateral L0,
+// pyramid and vertex maps at 3 levels.
+struct PreJob {
+    const uint16_t* raw;
+    float* depth_m;
+    float* d0;         // bilateral-filtered level 0
+    float4* v[3];      // camera-space vertex maps
+    int w, h;
+    float scale;
+    float Kinv[3][9];
+    int tile0, tiles_x;    // level-0 16x16 tiles of this job
+    int ptile0, ptiles_x;  // level-2 16x16 tiles of this job
+};
+int launch_preprocess_batch(const BatchHeader* hd, const PreJob* jobs, unsigned blocks, hipStream_t s);
+
 // ---- TSDF (tsdf.hip)
+constexpr unsigned kFuseBlocks = 2048;  // fuse workgroups: 256 CUs x 8 resident workgroups of 4 waves
+struct FuseArgs {
+    const float* depth;
+    int w, h;
+    float K[9];
+    float inv_pose[16];
+    const PoseState* dyn;  // when set: inv_pose comes from dyn->inv_pose, skip if dyn->frame_skip_fuse
+    int n, nb;
+    float half, res, mue;
+    const float* zk;
+    void* f;
+    void* wt;
+    unsigned long long* touched;
+    float* brick_min;
+    int* super_key;
+    int ns;
+    int cull;     // 1 when K is a plain pinhole matrix (culling planes valid)
+    int format;   // kf_voxel_format
+};
+struct FuseJob {
+    FuseArgs a;       // a.dyn set: the session's committed pose
+    unsigned brick0;  // first global brick index of this job
+};
+FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const float* K, const float* inv_pose,
+                   const PoseState* dyn_state);
+int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_blocks, unsigned list_blocks,
+                      hipStream_t s);
 // inv_pose: by-value world->camera matrix; dyn_state (may be null) overrides it
 // with PoseState::inv_pose and skips the launch's work when the frame failed.
 int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
@@ -41,11 +97,10 @@ struct RaycastLevel {
 struct RaycastJob {
     RaycastVolume vol;
     const float* pose;   // device: camera-to-world pose of the session (PoseState::pose)
-    const int* active;   // device flag, null = always active
     RaycastLevel L;
 };
 RaycastVolume raycast_volume(const kf_volume* v);
-int launch_raycast_batch(const RaycastJob* jobs_dev, int njobs, int tiles, hipStream_t s);
+int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s);
 int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
                    float4* vout, float4* nout, int w, int h, hipStream_t s);
 // All pyramid levels in one launch (level l is w0 >> l by h0 >> l).
@@ -55,6 +110,38 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
 
 // ---- ICP (icp.hip)
 constexpr int kPartialStride = 64;  // floats per block partial: 27 sums, 27 compensations, count
+struct IcpArgs {
+    const float4* V;
+    const float4* TV;
+    const float4* TN;
+    int w, h;
+    float K[9];
+    float dist_thresh, cos_thresh;
+    PoseState* st;  // pose source (R, t, Rp, tp) and early-stop flags
+    int level;
+    float* partials;
+    int stride;     // partial-row count of this launch (or of this job)
+};
+// State carried from one ICP launch to the next (double-buffered by launch parity).
+struct IcpCarry {
+    float R[9];
+    float t[3];
+    int status;
+    int level_done[4];
+    int iterations;
+    int pending;         // the writing launch left partial rows that still need their solve
+    int pending_level;
+    int pending_blocks;  // partial-row count of that launch
+    int pad[2];
+};
+// One session at one level of a batched ICP launch.
+struct IcpJob {
+    IcpArgs a;         // a.stride = the session's blocks at this level; a.partials unused
+    IcpCarry* carry;   // 2 entries, by launch parity
+    float* partials;   // 2 x kPartialStride x max_blocks, by launch parity
+    int max_blocks;
+    int block0;        // first block of this job in the launch
+};
 struct IcpLevel {
     const float4* V;
     const float4* TV;
@@ -69,5 +156,8 @@ int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream
 // begin: starts the frame's ICP from state->pose (or from pose16 if given);
 // launch_icp_loop then enqueues one launch per iteration and the final commit.
 int launch_icp_begin(kf_icp* icp, const float* pose16, hipStream_t s);
+// Batched: begin, then steps k = 0.. over the levels coarse to fine, then commit.
+int launch_icp_batch(const BatchHeader* hd, const IcpJob* jobs, const int* iters, int levels, int max_sessions,
+                     int compensated, float stop_angle, float stop_dist, unsigned step_blocks, hipStream_t s);
 
 }  // namespace kf

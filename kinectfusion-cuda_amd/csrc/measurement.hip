@@ -28,11 +28,37 @@ __global__ __launch_bounds__(256) void convert_kernel(const uint16_t* __restrict
 }
 
 // measurement.cu:33-66: 7x7, x outer / y inner, clamp-to-edge, sigma_s = 4, sigma_r = 0.25.
-// The 22x22 input window of the tile is staged in LDS once.
+// The 22x22 input window of the tile is staged in LDS once, filled with
+// clamp-to-edge reads, so tap (x, y) is simply the window element at the
+// thread's offset.
+typedef float BilateralTile[kTile + 6][kTile + 6 + 1];
+
+__device__ __forceinline__ float bilateral_at(const BilateralTile& tile, int tx, int ty)
+{
+    const float inv_s = 1.0f / (4.0f * 4.0f);
+    const float inv_r = 1.0f / (0.25f * 0.25f);
+    float center = tile[ty + 3][tx + 3];
+    float norm = 0.0f, acc = 0.0f;
+#pragma unroll
+    for (int x = -3; x <= 3; ++x) {
+#pragma unroll
+        for (int y = -3; y <= 3; ++y) {
+            float cur = tile[ty + 3 + y][tx + 3 + x];
+            float sd = (float)(x * x + y * y);
+            float id = center - cur;
+            id = id * id;
+            float f = kf_expf(-sd * inv_s - id * inv_r);
+            norm = norm + f;
+            acc = acc + f * cur;
+        }
+    }
+    return acc / norm;
+}
+
 __global__ __launch_bounds__(256) void bilateral_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
                                                         int h)
 {
-    __shared__ float tile[kTile + 6][kTile + 6 + 1];
+    __shared__ BilateralTile tile;
     const int i0 = blockIdx.x * kTile - 3, j0 = blockIdx.y * kTile - 3;
     const int t = threadIdx.y * kTile + threadIdx.x;
     for (int q = t; q < (kTile + 6) * (kTile + 6); q += kTile * kTile) {
@@ -43,63 +69,115 @@ __global__ __launch_bounds__(256) void bilateral_kernel(const float* __restrict_
     __syncthreads();
     int i = blockIdx.x * kTile + threadIdx.x, j = blockIdx.y * kTile + threadIdx.y;
     if (i >= w || j >= h) return;
-    const float inv_s = 1.0f / (4.0f * 4.0f);
-    const float inv_r = 1.0f / (0.25f * 0.25f);
-    // The LDS window was filled with clamp-to-edge reads, so tap (x, y) is
-    // simply the window element at the thread's offset.
-    float center = tile[threadIdx.y + 3][threadIdx.x + 3];
-    float norm = 0.0f, acc = 0.0f;
-#pragma unroll
-    for (int x = -3; x <= 3; ++x) {
-#pragma unroll
-        for (int y = -3; y <= 3; ++y) {
-            float cur = tile[threadIdx.y + 3 + y][threadIdx.x + 3 + x];
-            float sd = (float)(x * x + y * y);
-            float id = center - cur;
-            id = id * id;
-            float f = kf_expf(-sd * inv_s - id * inv_r);
-            norm = norm + f;
-            acc = acc + f * cur;
-        }
-    }
-    out[(int64_t)j * w + i] = acc / norm;
+    out[(int64_t)j * w + i] = bilateral_at(tile, threadIdx.x, threadIdx.y);
 }
 
-// measurement.cu:68-102
-__global__ __launch_bounds__(256) void down_sample_kernel(const float* __restrict__ in, int w, float* __restrict__ out,
-                                                          int wo, int ho)
+// measurement.cu:68-102: the taps within 3 sigma_r of the top-left tap, averaged.
+__device__ __forceinline__ float down4(float f1, float f2, float f3, float f4)
 {
-    int i = blockIdx.x * kTile + threadIdx.x, j = blockIdx.y * kTile + threadIdx.y;
-    if (i >= wo || j >= ho) return;
     const float tsd = 3.0f * 0.25f;
-    float f1 = in[(int64_t)(2 * j) * w + 2 * i], f2 = in[(int64_t)(2 * j) * w + 2 * i + 1];
-    float f3 = in[(int64_t)(2 * j + 1) * w + 2 * i], f4 = in[(int64_t)(2 * j + 1) * w + 2 * i + 1];
     float acc = f1;
     int cnt = 1;
     if (fabsf(f1 - f2) <= tsd) { acc = acc + f2; ++cnt; }
     if (fabsf(f1 - f3) <= tsd) { acc = acc + f3; ++cnt; }
     if (fabsf(f1 - f4) <= tsd) { acc = acc + f4; ++cnt; }
-    out[(int64_t)j * wo + i] = acc / (float)cnt;
+    return acc / (float)cnt;
+}
+
+__global__ __launch_bounds__(256) void down_sample_kernel(const float* __restrict__ in, int w, float* __restrict__ out,
+                                                          int wo, int ho)
+{
+    int i = blockIdx.x * kTile + threadIdx.x, j = blockIdx.y * kTile + threadIdx.y;
+    if (i >= wo || j >= ho) return;
+    float f1 = in[(int64_t)(2 * j) * w + 2 * i], f2 = in[(int64_t)(2 * j) * w + 2 * i + 1];
+    float f3 = in[(int64_t)(2 * j + 1) * w + 2 * i], f4 = in[(int64_t)(2 * j + 1) * w + 2 * i + 1];
+    out[(int64_t)j * wo + i] = down4(f1, f2, f3, f4);
+}
+
+// measurement.cu:104-126
+__device__ __forceinline__ float4 vertex_at(float d, int i, int j, const float* ki)
+{
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+    if (is_depth_valid(d)) {
+        v3 p = m3v(ki, mk3((float)i + 0.5f, (float)j + 0.5f, 1.0f));
+        p = mul3s(p, d);
+        o = make_float4(p.x, p.y, p.z, kValid);
+    }
+    return o;
 }
 
 struct Mat3 {
     float m[9];
 };
 
-// measurement.cu:104-126
 __global__ __launch_bounds__(256) void vertex_kernel(const float* __restrict__ depth, float4* __restrict__ out, int w,
                                                      int h, Mat3 ki)
 {
     int i = blockIdx.x * kTile + threadIdx.x, j = blockIdx.y * kTile + threadIdx.y;
     if (i >= w || j >= h) return;
-    float d = depth[(int64_t)j * w + i];
-    float4 o = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
-    if (is_depth_valid(d)) {
-        v3 p = m3v(ki.m, mk3((float)i + 0.5f, (float)j + 0.5f, 1.0f));
-        p = mul3s(p, d);
-        o = make_float4(p.x, p.y, p.z, kValid);
+    out[(int64_t)j * w + i] = vertex_at(depth[(int64_t)j * w + i], i, j, ki.m);
+}
+
+// ---- many sessions per launch (batch ticks), persistent over the tick's tiles.
+// Pass 1, per level-0 16x16 tile: raw codes -> metres in LDS (the fuse input is
+// written back), bilateral filter, level-0 vertex map.  Also clears the tick's
+// fuse-list counter, which the fuse passes of this tick use after it.
+__global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
+{
+    __shared__ BilateralTile tile;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hd->fuse_count = 0u;
+    const int tiles = hd->pre_tiles, njobs = hd->n_pre;
+    const int t = threadIdx.y * kTile + threadIdx.x;
+    int jb = 0;
+    for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
+        jb = job_of(jobs, njobs, tl, [](const PreJob& x) { return x.tile0; });
+        const PreJob& J = jobs[jb];
+        const int lt = tl - J.tile0;
+        const int i0 = (lt % J.tiles_x) * kTile - 3, j0 = (lt / J.tiles_x) * kTile - 3;
+        __syncthreads();  // the previous tile's reads of the window are over
+        for (int q = t; q < (kTile + 6) * (kTile + 6); q += kTile * kTile) {
+            int ly = q / (kTile + 6), lx = q % (kTile + 6);
+            int gi = clampi(i0 + lx, 0, J.w - 1), gj = clampi(j0 + ly, 0, J.h - 1);
+            tile[ly][lx] = half_rne(J.raw[(int64_t)gj * J.w + gi]) * J.scale;
+        }
+        __syncthreads();
+        const int i = i0 + 3 + threadIdx.x, j = j0 + 3 + threadIdx.y;
+        if (i < J.w && j < J.h) {
+            const int64_t q = (int64_t)j * J.w + i;
+            J.depth_m[q] = tile[threadIdx.y + 3][threadIdx.x + 3];
+            const float d = bilateral_at(tile, threadIdx.x, threadIdx.y);
+            J.d0[q] = d;
+            J.v[0][q] = vertex_at(d, i, j, J.Kinv[0]);
+        }
     }
-    out[(int64_t)j * w + i] = o;
+}
+
+// Pass 2, one thread per level-2 pixel: its 4x4 level-0 block gives 4 level-1
+// depths (downSample) and those the level-2 depth; vertex maps of both levels.
+__global__ __launch_bounds__(256) void pyramid_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
+{
+    const int tiles = hd->pyr_tiles, njobs = hd->n_pre;
+    for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
+        const PreJob& J = jobs[job_of(jobs, njobs, tl, [](const PreJob& x) { return x.ptile0; })];
+        const int lt = tl - J.ptile0;
+        const int i2 = (lt % J.ptiles_x) * kTile + threadIdx.x, j2 = (lt / J.ptiles_x) * kTile + threadIdx.y;
+        const int w1 = J.w >> 1, w2 = J.w >> 2;
+        if (i2 >= w2 || j2 >= (J.h >> 2)) continue;
+        float l1[2][2];
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int i1 = 2 * i2 + dx, j1 = 2 * j2 + dy;
+                const float* r0 = J.d0 + (int64_t)(2 * j1) * J.w + 2 * i1;
+                const float* r1 = r0 + J.w;
+                const float d = down4(r0[0], r0[1], r1[0], r1[1]);
+                l1[dy][dx] = d;
+                J.v[1][(int64_t)j1 * w1 + i1] = vertex_at(d, i1, j1, J.Kinv[1]);
+            }
+        const float d2 = down4(l1[0][0], l1[0][1], l1[1][0], l1[1][1]);
+        J.v[2][(int64_t)j2 * w2 + i2] = vertex_at(d2, i2, j2, J.Kinv[2]);
+    }
 }
 
 // measurement.cu:128-135 with device_helper.cuh:81-110
@@ -144,6 +222,15 @@ int launch_normal_map(const float4* v, float4* out, int w, int h, hipStream_t s)
 {
     normal_kernel<<<grid2d(w, h), dim3(kTile, kTile), 0, s>>>(v, out, w, h);
     KF_LAUNCHED("normal_kernel");
+    return KF_OK;
+}
+
+int launch_preprocess_batch(const BatchHeader* hd, const PreJob* jobs, unsigned blocks, hipStream_t s)
+{
+    measure_batch_kernel<<<blocks, dim3(kTile, kTile), 0, s>>>(hd, jobs);
+    KF_LAUNCHED("measure_batch_kernel");
+    pyramid_batch_kernel<<<blocks, dim3(kTile, kTile), 0, s>>>(hd, jobs);
+    KF_LAUNCHED("pyramid_batch_kernel");
     return KF_OK;
 }
 

@@ -263,27 +263,29 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
     raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j);
 }
 
-// Many sessions and levels in one launch: jobs[k] covers tiles
-// [jobs[k].L.tile0, jobs[k+1].L.tile0); a job whose session is inactive this
-// tick returns at once.
-__global__ __launch_bounds__(64) void raycast_batch_kernel(const RaycastJob* jobs, int njobs, int tiles)
+// Many sessions and levels in one persistent launch: jobs[k] covers tiles
+// [jobs[k].L.tile0, jobs[k+1].L.tile0) of the tick; workgroup ranks take
+// contiguous tile bands (XCD-contiguous) and stride through them.
+__global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
 {
-    const int tile = xcd_tile(tiles);
-    if (tile >= tiles) return;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].L.tile0 <= tile) lo = mid;
-        else hi = mid - 1;
+    const int tiles = hd->ray_tiles, njobs = hd->n_ray;
+    const int G = (int)gridDim.x;
+    const int rank = xcd_rank((int)blockIdx.x, G);
+    const int chunk = (tiles + G - 1) / G;
+    const int end = min(tiles, (rank + 1) * chunk);
+    int t = rank * chunk;
+    if (t >= end) return;
+    int jb = job_of(jobs, njobs, t, [](const RaycastJob& x) { return x.L.tile0; });
+    for (; t < end; ++t) {
+        while (jb + 1 < njobs && jobs[jb + 1].L.tile0 <= t) ++jb;
+        const RaycastJob& J = jobs[jb];
+        const int lt = t - J.L.tile0;
+        const int i = (lt % J.L.tiles_x) * 8 + (threadIdx.x & 7);
+        const int j = (lt / J.L.tiles_x) * 8 + (threadIdx.x >> 3);
+        if (i >= J.L.w || j >= J.L.h) continue;
+        if (J.vol.format == KF_VOXEL_F32) raycast_pixel<KF_VOXEL_F32>(J.vol, J.pose, J.L, i, j);
+        else raycast_pixel<KF_VOXEL_SNORM16_U8>(J.vol, J.pose, J.L, i, j);
     }
-    const RaycastJob& J = jobs[lo];
-    if (J.active && !*J.active) return;
-    const int lt = tile - J.L.tile0;
-    const int i = (lt % J.L.tiles_x) * 8 + (threadIdx.x & 7);
-    const int j = (lt / J.L.tiles_x) * 8 + (threadIdx.x >> 3);
-    if (i >= J.L.w || j >= J.L.h) return;
-    if (J.vol.format == KF_VOXEL_F32) raycast_pixel<KF_VOXEL_F32>(J.vol, J.pose, J.L, i, j);
-    else raycast_pixel<KF_VOXEL_SNORM16_U8>(J.vol, J.pose, J.L, i, j);
 }
 
 }  // namespace
@@ -303,10 +305,9 @@ RaycastVolume raycast_volume(const kf_volume* v)
     return r;
 }
 
-int launch_raycast_batch(const RaycastJob* jobs_dev, int njobs, int tiles, hipStream_t s)
+int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s)
 {
-    if (njobs <= 0 || tiles <= 0) return KF_OK;
-    raycast_batch_kernel<<<(unsigned)(((tiles + 7) / 8) * 8), 64, 0, s>>>(jobs_dev, njobs, tiles);
+    raycast_batch_kernel<<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
     return KF_OK;
 }

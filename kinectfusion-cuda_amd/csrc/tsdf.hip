@@ -22,27 +22,7 @@
 
 namespace kf {
 
-constexpr unsigned kFuseBlocks = 2048;  // fuse workgroups: 256 CUs x 8 resident workgroups of 4 waves
-
 namespace {
-
-struct FuseArgs {
-    const float* depth;
-    int w, h;
-    float K[9];
-    float inv_pose[16];
-    const PoseState* dyn;  // when set: inv_pose comes from dyn->inv_pose, skip if dyn->frame_skip_fuse
-    int n, nb;
-    float half, res, mue;
-    const float* zk;
-    void* f;
-    void* wt;
-    unsigned long long* touched;
-    float* brick_min;
-    int* super_key;
-    int ns;
-    int cull;  // 1 when K is a plain pinhole matrix (culling planes valid)
-};
 
 __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip, int bi, int bj, int bk)
 {
@@ -178,10 +158,49 @@ __global__ __launch_bounds__(kCullThreads) void fuse_cull_kernel(FuseArgs a, int
             (int)brick;
 }
 
-// Pass 2: waves stride over the visible bricks.  Each workgroup adds its count
-// of updated voxels to its own slot of `touched` (no atomics: launches on one
-// stream never run concurrently, and slot b belongs to workgroup b).
 constexpr int kFuseWaves = 4;
+
+// One wave fuses brick `brick` (lane = (j, i) inside the brick, 8 voxels along
+// z) and lowers the brick's and its super-brick's bound of F.  Returns the
+// lane's count of updated voxels.
+template <int FMT>
+__device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, int64_t brick, int lane)
+{
+    const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
+    const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+    const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
+    const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
+    const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
+    float fmin = __int_as_float(0x7f800000);
+    const int cnt = fuse_column<FMT>(a, ip, col, a.zk + bk * 8, brick * kBrickVoxels + lane, fmin);
+    // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) fmin = fminf(fmin, __shfl_xor(fmin, off, 64));
+    if (lane == 0 && fmin < a.brick_min[brick]) {
+        a.brick_min[brick] = fmin;
+        const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
+        atomicMin(&a.super_key[sidx], order_key(fmin));
+    }
+    return cnt;
+}
+
+// Sums the lanes' counts of a workgroup into its own slot (no atomics: launches
+// on one stream never run concurrently, and slot b belongs to workgroup b).
+__device__ __forceinline__ void add_touched(unsigned long long* touched, int cnt, int* wave_cnt)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if (lane == 0) wave_cnt[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) c += wave_cnt[w];
+        if (c) touched[blockIdx.x] += (unsigned long long)c;
+    }
+}
+
+// Pass 2: waves stride over the visible bricks.
 
 template <int FMT>
 __global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, const int* list,
@@ -195,33 +214,80 @@ __global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, 
     const unsigned int total = *count;
     const unsigned int waves = gridDim.x * kFuseWaves;
     int cnt = 0;
-    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves) {
-        const int64_t brick = list[item];
-        const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
-        const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
-        const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
-        const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
-        const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
-        float fmin = __int_as_float(0x7f800000);
-        cnt += fuse_column<FMT>(a, ip, col, a.zk + bk * 8, brick * kBrickVoxels + lane, fmin);
-        // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) fmin = fminf(fmin, __shfl_xor(fmin, off, 64));
-        if (lane == 0 && fmin < a.brick_min[brick]) {
-            a.brick_min[brick] = fmin;
-            const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
-            atomicMin(&a.super_key[sidx], order_key(fmin));
+    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves)
+        cnt += fuse_brick<FMT>(a, ip, list[item], lane);
+    add_touched(a.touched, cnt, wave_cnt);
+}
+
+// ---- many sessions per launch (batch ticks).  Jobs are sorted by brick0;
+// global brick g of the tick belongs to the job whose range holds it.
+constexpr int kBatchCullThreads = 256;
+
+__global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(const BatchHeader* hd,
+                                                                           const FuseJob* jobs)
+{
+    __shared__ unsigned int wave_base[kBatchCullThreads / 64 + 1];
+    const unsigned total = hd->fuse_bricks;
+    const int njobs = hd->n_fuse;
+    const unsigned stride = gridDim.x * kBatchCullThreads;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // uniform trip count so every lane reaches the barriers
+    for (unsigned base = blockIdx.x * kBatchCullThreads; base < total; base += stride) {
+        const unsigned g = base + threadIdx.x;
+        bool vis = false;
+        if (g < total) {
+            const FuseJob& J = jobs[job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; })];
+            const FuseArgs& a = J.a;
+            if (!a.dyn->frame_skip_fuse) {
+                const float* ip = a.dyn->inv_pose;
+                const int64_t brick = g - J.brick0;
+                const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
+                const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+                vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
+            }
         }
+        const uint64_t m = __ballot(vis);
+        if (lane == 0) wave_base[wave] = (unsigned int)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int t = 0;
+            for (int w = 0; w < kBatchCullThreads / 64; ++w) {
+                unsigned int c = wave_base[w];
+                wave_base[w] = t;
+                t += c;
+            }
+            wave_base[kBatchCullThreads / 64] = t ? atomicAdd(hd->fuse_count, t) : 0u;
+        }
+        __syncthreads();
+        if (vis)
+            hd->fuse_list[wave_base[kBatchCullThreads / 64] + wave_base[wave] +
+                          (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = g;
+        __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(64 * kFuseWaves) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
+{
+    __shared__ int wave_cnt[kFuseWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned int total = *hd->fuse_count;
+    const unsigned int waves = gridDim.x * kFuseWaves;
+    const int njobs = hd->n_fuse;
+    int cnt = 0, cur = -1;
+    float ip[16];
+    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves) {
+        const unsigned g = hd->fuse_list[item];
+        const int j = job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; });
+        const FuseJob& J = jobs[j];
+        if (j != cur) {
+            cur = j;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) wave_cnt[wave] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int c = 0;
-        for (int w = 0; w < kFuseWaves; ++w) c += wave_cnt[w];
-        if (c) a.touched[blockIdx.x] += (unsigned long long)c;
+            for (int q = 0; q < 16; ++q) ip[q] = J.a.dyn->inv_pose[q];
+        }
+        if (J.a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(J.a, ip, g - J.brick0, lane);
+        else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(J.a, ip, g - J.brick0, lane);
     }
+    add_touched(hd->touched, cnt, wave_cnt);
 }
 
 template <int FMT>
@@ -369,8 +435,8 @@ int for_each_slab(const kf_volume* v, bool to_volume, Fn&& host_side)
 
 }  // namespace
 
-int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
-                const PoseState* dyn_state, hipStream_t s)
+FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const float* K, const float* inv_pose,
+                   const PoseState* dyn_state)
 {
     FuseArgs a{};
     a.depth = depth;
@@ -392,6 +458,24 @@ int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, 
     a.super_key = v->super_key;
     a.ns = v->ns;
     a.cull = is_pinhole(K) ? 1 : 0;
+    a.format = v->format;
+    return a;
+}
+
+int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_blocks, unsigned list_blocks,
+                      hipStream_t s)
+{
+    fuse_batch_cull_kernel<<<cull_blocks, kBatchCullThreads, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("fuse_batch_cull_kernel");
+    fuse_batch_list_kernel<<<list_blocks, 64 * kFuseWaves, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("fuse_batch_list_kernel");
+    return KF_OK;
+}
+
+int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
+                const PoseState* dyn_state, hipStream_t s)
+{
+    FuseArgs a = fuse_args(depth, w, h, v, K, inv_pose, dyn_state);
     const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
     const int parity = (int)(v->fuse_launches & 1);
     fuse_cull_kernel<<<(unsigned)((nbricks + kCullThreads - 1) / kCullThreads), kCullThreads, 0, s>>>(

@@ -15,13 +15,13 @@ import numpy as np
 
 from . import _capi
 from ._capi import (KF_ERR_ARG, KF_ERR_DIMS, KF_ERR_IO, KF_ERR_NONFINITE, KF_ERR_NOT_PD, KF_OK,
-                    KF_VOXEL_F32, KF_VOXEL_SNORM16_U8, STAGE_NAMES, kf_icp_config, kf_pipeline_config,
-                    kf_sensor)
+                    KF_VOXEL_F32, KF_VOXEL_SNORM16_U8, STAGE_NAMES, kf_batch_config, kf_batch_shape,
+                    kf_icp_config, kf_pipeline_config, kf_session_config, kf_sensor)
 
 __all__ = [
     "KfError", "lib", "init", "DeviceArray", "GridMap", "DepthMap", "GridMapPyramid", "Sensor",
     "RigidTransform3D", "VoxelGrid", "ICP", "IcpConfig", "LinearLeastSquares", "kernel", "Pipeline",
-    "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
+    "Batch", "SessionConfig", "session_bytes", "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
 ]
 
 
@@ -491,6 +491,124 @@ class Pipeline:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().kf_pipeline_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def SessionConfig(width=640, height=480, fx=525.0, fy=525.0, cx=None, cy=None, depth_scale=1e-3, volume_n=512,
+                  volume_size=4.0, mue=0.03, voxel_format=KF_VOXEL_SNORM16_U8, initial_pose=None):
+    """kf_session_config; cx, cy default to (w - 1) / 2, (h - 1) / 2 (sensor.h:14 for 640x480)."""
+    c = kf_session_config()
+    c.frame_width, c.frame_height = width, height
+    c.depth_scale = depth_scale
+    c.fx, c.fy = fx, fy
+    c.cx = (width - 1) / 2.0 if cx is None else cx
+    c.cy = (height - 1) / 2.0 if cy is None else cy
+    c.volume_n, c.volume_size, c.mue, c.voxel_format = volume_n, volume_size, mue, voxel_format
+    pose = np.eye(4, dtype=np.float32).T.reshape(-1) if initial_pose is None else \
+        np.asarray(initial_pose, np.float32).reshape(-1)
+    for i in range(16):
+        c.initial_pose[i] = float(pose[i])
+    return c
+
+
+def session_bytes(cfg):
+    n = C.c_size_t()
+    check(lib().kf_session_bytes(C.byref(cfg), C.byref(n)), "kf_session_bytes")
+    return n.value
+
+
+class Batch:
+    """Many sessions advanced together, one launch per stage per tick (kf_batch_*)."""
+
+    def __init__(self, max_sessions, icp=None, use_graph=True, autotune=False, shape=None, stream=None):
+        init()
+        c = kf_batch_config()
+        c.max_sessions = max_sessions
+        c.icp = icp if icp is not None else IcpConfig()
+        c.use_graph = int(use_graph)
+        c.autotune = int(autotune)
+        if shape:
+            for k, v in shape.items():
+                setattr(c.shape, k, int(v))
+        self.cfg = c
+        h = C.c_void_p()
+        check(lib().kf_batch_create(C.byref(c), stream, C.byref(h)), "kf_batch_create")
+        self.h = h
+        self.sessions = {}
+
+    def add(self, cfg):
+        sid = C.c_int()
+        check(lib().kf_batch_add_session(self.h, C.byref(cfg), C.byref(sid)), "kf_batch_add_session")
+        self.sessions[sid.value] = cfg
+        return sid.value
+
+    def remove(self, sid):
+        check(lib().kf_batch_remove_session(self.h, sid), "kf_batch_remove_session")
+        self.sessions.pop(sid, None)
+
+    def tick(self, ids, raw_ptrs):
+        n = len(ids)
+        ia = (C.c_int * max(n, 1))(*ids)
+        pa = (C.c_void_p * max(n, 1))(*[int(p) for p in raw_ptrs])
+        check(lib().kf_batch_tick(self.h, n, ia, pa), "kf_batch_tick")
+
+    def sync(self):
+        check(lib().kf_batch_sync(self.h), "kf_batch_sync")
+
+    def pose(self, sid):
+        p = np.zeros(16, np.float32)
+        check(lib().kf_batch_pose(self.h, sid, _p(p)), "kf_batch_pose")
+        return p
+
+    def status(self, sid):
+        st = C.c_int()
+        check(lib().kf_batch_session_status(self.h, sid, C.byref(st)), "kf_batch_session_status")
+        return st.value
+
+    def map(self, sid, which, level=0):
+        ptr, w, h = C.c_void_p(), C.c_int(), C.c_int()
+        check(lib().kf_batch_map(self.h, sid, which, level, C.byref(ptr), C.byref(w), C.byref(h)), "kf_batch_map")
+        ch = 1 if which in (0, 1) else 4
+        shape = (h.value, w.value) if ch == 1 else (h.value, w.value, ch)
+        out = np.empty(shape, np.float32)
+        self.sync()
+        check(lib().kf_memcpy_d2h(_p(out), ptr, out.nbytes, None), "map download")
+        return out
+
+    def volume_download(self, sid):
+        v = C.c_void_p()
+        self.sync()
+        check(lib().kf_batch_volume(self.h, sid, C.byref(v)), "kf_batch_volume")
+        n = self.sessions[sid].volume_n
+        fw = np.empty(n ** 3 * 2, np.float32)
+        check(lib().kf_volume_download(v, _p(fw)), "kf_volume_download")
+        return fw[0::2].copy(), fw[1::2].copy()
+
+    def shape(self):
+        sh = kf_batch_shape()
+        tuned = C.c_int()
+        check(lib().kf_batch_shape_get(self.h, C.byref(sh), C.byref(tuned)), "kf_batch_shape_get")
+        return {k: getattr(sh, k) for k, _ in sh._fields_}, bool(tuned.value)
+
+    def tick_times(self):
+        tot, n, mx = C.c_double(), C.c_int64(), C.c_float()
+        check(lib().kf_batch_tick_times(self.h, C.byref(tot), C.byref(n), C.byref(mx)), "kf_batch_tick_times")
+        return tot.value, n.value, mx.value
+
+    def touched_total(self):
+        v = C.c_ulonglong()
+        check(lib().kf_batch_touched_total(self.h, C.byref(v)), "kf_batch_touched_total")
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().kf_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -72,6 +72,43 @@ class kf_pipeline_config(C.Structure):
     ]
 
 
+class kf_session_config(C.Structure):
+    _fields_ = [
+        ("frame_width", C.c_int),
+        ("frame_height", C.c_int),
+        ("depth_scale", C.c_float),
+        ("fx", C.c_float),
+        ("fy", C.c_float),
+        ("cx", C.c_float),
+        ("cy", C.c_float),
+        ("volume_n", C.c_int),
+        ("volume_size", C.c_float),
+        ("mue", C.c_float),
+        ("voxel_format", C.c_int),
+        ("initial_pose", C.c_float * 16),
+    ]
+
+
+class kf_batch_shape(C.Structure):
+    _fields_ = [
+        ("preprocess_blocks", C.c_uint),
+        ("raycast_blocks", C.c_uint),
+        ("icp_blocks", C.c_uint),
+        ("cull_blocks", C.c_uint),
+        ("fuse_blocks", C.c_uint),
+    ]
+
+
+class kf_batch_config(C.Structure):
+    _fields_ = [
+        ("max_sessions", C.c_int),
+        ("icp", kf_icp_config),
+        ("use_graph", C.c_int),
+        ("autotune", C.c_int),
+        ("shape", kf_batch_shape),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 F = C.c_float
@@ -136,6 +173,20 @@ SIGNATURES = {
     "kf_pipeline_set_timing": (I, [P, I]),
     "kf_pipeline_stage_times": (I, [P, P, C.POINTER(I64)]),
     "kf_pipeline_fuse_kernel_ms": (I, [P, FP, C.POINTER(I64)]),
+    "kf_session_bytes": (I, [C.POINTER(kf_session_config), C.POINTER(SZ)]),
+    "kf_batch_create": (I, [C.POINTER(kf_batch_config), P, C.POINTER(P)]),
+    "kf_batch_destroy": (I, [P]),
+    "kf_batch_add_session": (I, [P, C.POINTER(kf_session_config), C.POINTER(I)]),
+    "kf_batch_remove_session": (I, [P, I]),
+    "kf_batch_tick": (I, [P, I, P, P]),
+    "kf_batch_sync": (I, [P]),
+    "kf_batch_pose": (I, [P, I, P]),
+    "kf_batch_session_status": (I, [P, I, C.POINTER(I)]),
+    "kf_batch_volume": (I, [P, I, C.POINTER(P)]),
+    "kf_batch_map": (I, [P, I, I, I, C.POINTER(P), C.POINTER(I), C.POINTER(I)]),
+    "kf_batch_shape_get": (I, [P, C.POINTER(kf_batch_shape), C.POINTER(I)]),
+    "kf_batch_tick_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64), FP]),
+    "kf_batch_touched_total": (I, [P, C.POINTER(C.c_ulonglong)]),
 }
 
 _lib = None
