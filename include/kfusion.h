@@ -234,7 +234,8 @@ int kf_pipeline_map(kf_pipeline* p, int which, int level, void** dev_ptr, int* w
 /* Per-stage event timing (kinect_fusion.cpp:216-224 prints these averages). */
 int kf_pipeline_set_timing(kf_pipeline* p, int enabled);
 int kf_pipeline_stage_times(kf_pipeline* p, float total_ms[KF_STAGE_COUNT], int64_t* frames);
-/* Average duration of the fuse launches timed since timing was enabled (ms). */
+/* Average duration of the fuse's brick-list kernel (the voxel sweep) over the frames timed since
+ * timing was enabled (ms), from events recorded around that kernel alone. */
 int kf_pipeline_fuse_kernel_ms(kf_pipeline* p, float* avg_ms, int64_t* launches);
 
 /* ---------------------------------------------------------------- batch ---

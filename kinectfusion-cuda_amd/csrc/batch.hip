@@ -37,6 +37,7 @@ struct Session {
     int max_blocks = 0;
     float* depth_m = nullptr;
     float* d0 = nullptr;
+    float* dmax = nullptr;
     float4* v[3] = {nullptr, nullptr, nullptr};
     float4* pv[3] = {nullptr, nullptr, nullptr};
     float4* pn[3] = {nullptr, nullptr, nullptr};
@@ -65,6 +66,33 @@ const unsigned kCand[4][kCandidates] = {
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+__global__ void fill_invalid_kernel(float4* m, int64_t n)
+{
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) m[q] = make_float4(0.0f, 0.0f, 0.0f, kf::kInvalid);
+}
+
+struct Pose32 {
+    float m[32];  // pose, inverse pose
+};
+// A new session's PoseState (pose and its inverse, everything else zero) and
+// zeroed ICP carries.
+__global__ void pose_state_init_kernel(kf::PoseState* st, kf::IcpCarry* carry, Pose32 p)
+{
+    if (threadIdx.x != 0) return;
+    kf::PoseState z;
+    memset(&z, 0, sizeof(z));
+    for (int q = 0; q < 16; ++q) {
+        z.pose[q] = p.m[q];
+        z.inv_pose[q] = p.m[16 + q];
+    }
+    *st = z;
+    kf::IcpCarry c;
+    memset(&c, 0, sizeof(c));
+    carry[0] = c;
+    carry[1] = c;
+}
 
 }  // namespace
 
@@ -111,7 +139,7 @@ size_t volume_meta_bytes(int n)
 
 size_t maps_bytes(int w, int h)
 {
-    size_t px = (size_t)w * h, b = 2 * sizeof(float) * px;  // metres + filtered level 0
+    size_t px = (size_t)w * h, b = 2 * sizeof(float) * px + sizeof(float) * kf::depth_max_floats(w, h);
     for (int l = 0; l < 3; ++l) b += 3 * sizeof(float4) * (size_t)(w >> l) * (h >> l);
     return b + 2 * sizeof(kf::IcpCarry) + 2 * sizeof(float) * kf::kPartialStride * (size_t)kf::icp_blocks_for(w, h) +
            sizeof(kf::PoseState);
@@ -135,6 +163,7 @@ void free_session(Session& q)
     (void)hipFree(q.partials);
     (void)hipFree(q.depth_m);
     (void)hipFree(q.d0);
+    (void)hipFree(q.dmax);
     for (int l = 0; l < 3; ++l) {
         (void)hipFree(q.v[l]);
         (void)hipFree(q.pv[l]);
@@ -160,6 +189,7 @@ int ensure_list(kf_batch* b)
     for (const Session& q : b->sess)
         if (q.used) need += (size_t)(q.cfg.volume_n / 8) * (q.cfg.volume_n / 8) * (q.cfg.volume_n / 8);
     if (need <= b->list_cap) return KF_OK;
+    need = std::max(need, 2 * b->list_cap);  // grow geometrically: each growth waits for the ticks in flight
     KF_HIP(hipStreamSynchronize(b->s));
     (void)hipFree(b->fuse_list);
     b->fuse_list = nullptr;
@@ -405,7 +435,9 @@ int kf_batch_add_session(kf_batch* b, const kf_session_config* c, int* id)
     q.cfg = *c;
     const int w = c->frame_width, h = c->frame_height;
     kf_sensor sensor;
-    st = kf_volume_create(c->volume_n, c->volume_size, c->mue, c->voxel_format, &q.vol);
+    // everything is initialised on the batch's stream, so admitting a session
+    // does not wait for the ticks in flight
+    st = kf::volume_create(c->volume_n, c->volume_size, c->mue, c->voxel_format, b->s, &q.vol);
     if (!st) st = kf_sensor_init(&sensor, c->fx, c->fy, c->cx, c->cy);
     if (!st) st = kf_sensor_set_pose(&sensor, c->initial_pose);
     if (!st) {
@@ -419,24 +451,25 @@ int kf_batch_add_session(kf_batch* b, const kf_session_config* c, int* id)
     if (!st) st = alloc((void**)&q.partials, 2 * sizeof(float) * kf::kPartialStride * (size_t)q.max_blocks, &q.bytes);
     if (!st) st = alloc((void**)&q.depth_m, sizeof(float) * (size_t)w * h, &q.bytes);
     if (!st) st = alloc((void**)&q.d0, sizeof(float) * (size_t)w * h, &q.bytes);
+    if (!st) st = alloc((void**)&q.dmax, sizeof(float) * kf::depth_max_floats(w, h), &q.bytes);
     for (int l = 0; l < 3 && !st; ++l) {
         const size_t px = (size_t)(w >> l) * (h >> l);
         if ((st = alloc((void**)&q.v[l], sizeof(float4) * px, &q.bytes))) break;
         if ((st = alloc((void**)&q.pv[l], sizeof(float4) * px, &q.bytes))) break;
         if ((st = alloc((void**)&q.pn[l], sizeof(float4) * px, &q.bytes))) break;
         // predicted maps start invalid, as kf_pipeline's
-        std::vector<float4> inval(px, make_float4(0.0f, 0.0f, 0.0f, kf::kInvalid));
-        hipError_t e = hipMemcpy(q.pv[l], inval.data(), sizeof(float4) * px, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(q.pn[l], inval.data(), sizeof(float4) * px, hipMemcpyHostToDevice);
+        const unsigned blocks = (unsigned)((px + 255) / 256);
+        fill_invalid_kernel<<<blocks, 256, 0, b->s>>>(q.pv[l], (int64_t)px);
+        fill_invalid_kernel<<<blocks, 256, 0, b->s>>>(q.pn[l], (int64_t)px);
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = kf::fail_hip(e, "kf_batch_add_session init maps");
     }
     if (!st) {
-        kf::PoseState hs;
-        memset(&hs, 0, sizeof(hs));
-        memcpy(hs.pose, sensor.pose, sizeof(hs.pose));
-        memcpy(hs.inv_pose, sensor.inv_pose, sizeof(hs.inv_pose));
-        hipError_t e = hipMemcpy(q.st, &hs, sizeof(hs), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(q.carry, 0, 2 * sizeof(kf::IcpCarry));
+        Pose32 p;
+        memcpy(p.m, sensor.pose, sizeof(sensor.pose));
+        memcpy(p.m + 16, sensor.inv_pose, sizeof(sensor.inv_pose));
+        pose_state_init_kernel<<<1, 64, 0, b->s>>>(q.st, q.carry, p);
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = kf::fail_hip(e, "kf_batch_add_session state");
     }
     if (st) {
@@ -507,6 +540,7 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
         P.raw = raw[k];
         P.depth_m = q.depth_m;
         P.d0 = q.d0;
+        P.dmax = q.dmax;
         for (int l = 0; l < 3; ++l) P.v[l] = q.v[l];
         P.w = w;
         P.h = h;
@@ -519,22 +553,8 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
         P.ptiles_x = (w / 4 + 15) / 16;
         pyr_tiles += P.ptiles_x * ((h / 4 + 15) / 16);
         if (q.frames > 0) {
-            // tracking: raycast L0..L2 at the previous pose, then ICP
-            for (int l = 0; l < ic.levels; ++l) {
-                kf::RaycastJob& R = ray[n_ray++];
-                memset(&R, 0, sizeof(R));
-                R.vol = kf::raycast_volume(q.vol);
-                R.pose = reinterpret_cast<const float*>(reinterpret_cast<const char*>(q.st) +
-                                                        offsetof(kf::PoseState, pose));
-                memcpy(R.L.Kinv, q.Ki[l], sizeof(R.L.Kinv));
-                R.L.vout = q.pv[l];
-                R.L.nout = q.pn[l];
-                R.L.w = w >> l;
-                R.L.h = h >> l;
-                R.L.tiles_x = (R.L.w + 7) / 8;
-                R.L.tile0 = ray_tiles;
-                ray_tiles += R.L.tiles_x * ((R.L.h + 7) / 8);
-            }
+            // tracking: raycast L0..L2 at the previous pose (jobs added below, FP32
+            // volumes first), then ICP
             for (int l = 0; l < ic.levels; ++l) {
                 kf::IcpJob& J = icp[(size_t)l * m + n_icp];
                 memset(&J, 0, sizeof(J));
@@ -560,9 +580,38 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
         kf::FuseJob& F = fuse[k];
         memset(&F, 0, sizeof(F));
         F.a = kf::fuse_args(q.depth_m, w, h, q.vol, q.K[0], nullptr, q.st);
+        F.a.dmax = q.dmax;
         F.brick0 = bricks;
         const unsigned nb = (unsigned)(q.cfg.volume_n / 8);
         bricks += nb * nb * nb;
+    }
+    // raycast jobs of the tracking sessions, FP32 volumes first (one launch per format)
+    for (int pass = 0; pass < 2; ++pass) {
+        const int fmt = pass == 0 ? KF_VOXEL_F32 : KF_VOXEL_SNORM16_U8;
+        if (pass == 1) {
+            hd->n_ray_f32 = n_ray;
+            hd->ray_tiles_f32 = ray_tiles;
+        }
+        for (int k = 0; k < n; ++k) {
+            Session& q = b->sess[ids[k]];
+            if (q.frames == 0 || q.cfg.voxel_format != fmt) continue;
+            const int w = q.cfg.frame_width, h = q.cfg.frame_height;
+            for (int l = 0; l < ic.levels; ++l) {
+                kf::RaycastJob& R = ray[n_ray++];
+                memset(&R, 0, sizeof(R));
+                R.vol = kf::raycast_volume(q.vol);
+                R.pose = reinterpret_cast<const float*>(reinterpret_cast<const char*>(q.st) +
+                                                        offsetof(kf::PoseState, pose));
+                memcpy(R.L.Kinv, q.Ki[l], sizeof(R.L.Kinv));
+                R.L.vout = q.pv[l];
+                R.L.nout = q.pn[l];
+                R.L.w = w >> l;
+                R.L.h = h >> l;
+                R.L.tiles_x = (R.L.w + 7) / 8;
+                R.L.tile0 = ray_tiles;
+                ray_tiles += R.L.tiles_x * ((R.L.h + 7) / 8);
+            }
+        }
     }
     hd->n_pre = n;
     hd->pre_tiles = pre_tiles;

@@ -290,45 +290,126 @@ __global__ __launch_bounds__(kThreads) void icp_rows_kernel(IcpArgs a, float* A,
     B[px] = b;
 }
 
-// ---- FP64 6x6 Cholesky, unblocked, in the column order of Eigen's LLT
-// (3rd/eigen/Eigen/src/Cholesky/LLT.h:306-326); 0 or the failing column + 1.
-__device__ int chol6(const double* ata, const double* atb, double* x)
+// ---- FP64 6x6 Cholesky solve, unblocked, in the column order of Eigen's LLT
+// (3rd/eigen/Eigen/src/Cholesky/LLT.h:306-326), by one wave: lane r < 6
+// holds row r of A (from the 27 sums) and of L; column k is finished by its
+// lanes and its values are broadcast with readlane.  Every element is computed
+// by the same FP64 operations in the same order as the sequential form
+//   L(k,k) = sqrt(A(k,k) - sum_j<k L(k,j)^2),
+//   L(r,k) = (A(r,k) - sum_j<k L(r,j) L(k,j)) / L(k,k),
+//   y(r) = (b(r) - sum_j<r L(r,j) y(j)) / L(r,r),
+//   x(r) = (y(r) - sum_j>r L(j,r) x(j)) / L(r,r),
+// so the result is the same, while no lane holds the whole matrix (a one-lane
+// FP64 solve would set the register budget of every kernel that embeds it).
+// Called by all 64 lanes of a wave; returns 0 or the failing column + 1 and
+// the solution in x (the same values in every lane).
+__device__ __forceinline__ double bcast(double v, int lane)
 {
-    double L[6][6];
-    for (int c = 0; c < 6; ++c)
-        for (int r = 0; r < 6; ++r) L[r][c] = ata[c * 6 + r];
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// index of A(p, q), p <= q, among the 21 sums: rows of the upper triangle
+__device__ __forceinline__ int sym_index(int p, int q) { return p * 6 - (p * (p - 1)) / 2 + (q - p); }
+
+__device__ int chol6_wave(const double* tot, double* x)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane < 6 ? lane : 5;  // lanes >= 6 shadow row 5; their values are never read
+    double a[6], Lr[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        a[c] = tot[sym_index(min(r, c), max(r, c))];
+        Lr[c] = 0.0;
+    }
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
-        double d = L[k][k];
-        for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
+        double d = a[k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) d -= Lr[j] * Lr[j];
+        d = bcast(d, k);
         if (!(d > 0.0)) return k + 1;
         d = sqrt(d);
-        L[k][k] = d;
-        for (int r = k + 1; r < 6; ++r) {
-            double v = L[r][k];
-            for (int j = 0; j < k; ++j) v -= L[r][j] * L[k][j];
-            L[r][k] = v / d;
+        double Lk[6];
+#pragma unroll
+        for (int j = 0; j < k; ++j) Lk[j] = bcast(Lr[j], k);
+        if (r == k) {
+            Lr[k] = d;
+        } else if (r > k) {
+            double v = a[k];
+#pragma unroll
+            for (int j = 0; j < k; ++j) v -= Lr[j] * Lk[j];
+            Lr[k] = v / d;
         }
     }
     double y[6];
-    for (int r = 0; r < 6; ++r) {
-        double v = atb[r];
-        for (int j = 0; j < r; ++j) v -= L[r][j] * y[j];
-        y[r] = v / L[r][r];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double v = tot[21 + r];
+#pragma unroll
+        for (int j = 0; j < q; ++j) v -= Lr[j] * y[j];
+        y[q] = bcast(v / Lr[r], q);
     }
-    for (int r = 5; r >= 0; --r) {
-        double v = y[r];
-        for (int j = r + 1; j < 6; ++j) v -= L[j][r] * x[j];
-        x[r] = v / L[r][r];
+#pragma unroll
+    for (int q = 5; q >= 0; --q) {
+        double v = y[q];
+#pragma unroll
+        for (int j = q + 1; j < 6; ++j) v -= bcast(Lr[q], j) * x[j];
+        x[q] = v / bcast(Lr[q], q);
     }
     return 0;
 }
 
 // icp.cpp:65-87 in FP64: R_inc = R_y(x1) R_x(x0) R_z(x2) (glm orientate3(x0, x2, x1)),
 // R <- R_inc^T R, t <- R_inc^T t + x[3..5].
+//
+// sin and cos in FP64 of an angle x: Cody-Waite reduction r = x - k pi/2 with
+// pi/2 split in a 33-bit head (k * head exact for |k| < 2^20) and a tail,
+// then the Taylor series of sin and cos on |r| <= pi/4 (terms through r^17
+// and r^18: the next terms are below 2^-60 there).  Accurate to a few ulp of
+// double for |x| < 2^20 pi/2, far beyond any ICP update; the library routines
+// carry a huge-argument reduction whose registers every lane of the kernel
+// would pay for.
+__device__ __forceinline__ void sincos_small(double x, double* sn, double* cs)
+{
+    const double kHead = 1.57079632673412561417e+00;  // leading 33 bits of pi/2
+    const double kTail = 6.07710050650619224932e-11;  // pi/2 - kHead
+    const double k = rint(x * 0.636619772367581343076);
+    const double r = (x - k * kHead) - k * kTail;
+    const double z = r * r;
+    double ps = 1.0 / 355687428096000.0;       // 1/17!
+    ps = ps * z - 1.0 / 1307674368000.0;       // 1/15!
+    ps = ps * z + 1.0 / 6227020800.0;          // 1/13!
+    ps = ps * z - 1.0 / 39916800.0;            // 1/11!
+    ps = ps * z + 1.0 / 362880.0;              // 1/9!
+    ps = ps * z - 1.0 / 5040.0;                // 1/7!
+    ps = ps * z + 1.0 / 120.0;                 // 1/5!
+    ps = ps * z - 1.0 / 6.0;                   // 1/3!
+    const double s = r + r * (z * ps);
+    double pc = 1.0 / 6402373705728000.0;      // 1/18!
+    pc = pc * z - 1.0 / 20922789888000.0;      // 1/16!
+    pc = pc * z + 1.0 / 87178291200.0;         // 1/14!
+    pc = pc * z - 1.0 / 479001600.0;           // 1/12!
+    pc = pc * z + 1.0 / 3628800.0;             // 1/10!
+    pc = pc * z - 1.0 / 40320.0;               // 1/8!
+    pc = pc * z + 1.0 / 720.0;                 // 1/6!
+    pc = pc * z - 1.0 / 24.0;                  // 1/4!
+    pc = pc * z + 0.5;                         // 1/2!
+    const double c = 1.0 - z * pc;
+    const int q = (int)((long long)k & 3);
+    *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
 __device__ void update_pose(float* Rm, float* t, const float* x)
 {
     double pitch = x[0], yaw = x[1], roll = x[2];
-    double ch = cos(yaw), sh = sin(yaw), cp = cos(pitch), sp = sin(pitch), cb = cos(roll), sb = sin(roll);
+    double ch, sh, cp, sp, cb, sb;
+    sincos_small(yaw, &sh, &ch);
+    sincos_small(pitch, &sp, &cp);
+    sincos_small(roll, &sb, &cb);
     double Ri[9];
     Ri[0] = ch * cb + sh * sp * sb;  Ri[1] = sb * cp;  Ri[2] = -sh * cb + ch * sp * sb;
     Ri[3] = -ch * sb + sh * sp * cb; Ri[4] = cb * cp;  Ri[5] = sb * sh + ch * sp * cb;
@@ -390,38 +471,25 @@ __device__ void sum_partials(const float* partials, int stride, int nblocks, dou
     __syncthreads();
 }
 
-__device__ void expand(const double* tot, double* ata, double* atb)
+// Pose update of one iteration from its FP64 solution (one lane): the float
+// update x, the pose update (icp.cpp:65-87) and the early-stop test
+// (icp.cpp:45-53).  Returns 0 or KF_ERR_NONFINITE; *done = early stop.
+__device__ int pose_step(const double* xd, float* R, float* t, float stop_angle, float stop_dist, float* x,
+                         bool* done)
 {
-    int q = 0;
-    for (int p = 0; p < 6; ++p)
-        for (int r = p; r < 6; ++r, ++q) {
-            ata[r * 6 + p] = tot[q];
-            ata[p * 6 + r] = tot[q];
-        }
-    for (int p = 0; p < 6; ++p) atb[p] = tot[21 + p];
-}
-
-// Solve of one iteration from its 27 sums (thread 0): FP64 Cholesky, pose
-// update (icp.cpp:65-87) and the early-stop test (icp.cpp:45-53).  Returns 0 or
-// a kf_status; x receives the float update.
-__device__ int solve_update(const double* tot, float* R, float* t, int* level_done, float stop_angle,
-                            float stop_dist, float* x)
-{
-    double ata[36], atb[6], xd[6];
-    expand(tot, ata, atb);
-    if (chol6(ata, atb, xd) != 0) return KF_ERR_NOT_PD;
     bool finite = true;
     for (int q = 0; q < 6; ++q) {
         x[q] = (float)xd[q];
         finite = finite && isfinite(x[q]);
     }
+    *done = false;
     if (!finite) return KF_ERR_NONFINITE;
     float Rprev[9], tprev[3], err[2];
     for (int q = 0; q < 9; ++q) Rprev[q] = R[q];
     for (int q = 0; q < 3; ++q) tprev[q] = t[q];
     update_pose(R, t, x);
     pose_error(Rprev, tprev, R, t, err);
-    if (err[0] < stop_angle && err[1] < stop_dist) *level_done = 1;
+    *done = err[0] < stop_angle && err[1] < stop_dist;
     return KF_OK;
 }
 
@@ -434,12 +502,13 @@ __device__ void finalize_body(PoseState* st, const float* partials, int stride, 
     __shared__ double tot[kTerms];
     __shared__ double count;
     sum_partials<COMP>(partials, stride, nblocks, tot, &count);
+    if (threadIdx.x >= 64) return;
+    double xd[6];
+    const int rc = chol6_wave(tot, xd);
     if (threadIdx.x != 0) return;
     for (int q = 0; q < kTerms; ++q) st->last_sums[q] = tot[q];
     st->last_inliers = (long long)count;
-    double ata[36], atb[6], xd[6];
-    expand(tot, ata, atb);
-    if (chol6(ata, atb, xd) != 0) {
+    if (rc != 0) {
         if (st->status == 0) st->status = KF_ERR_NOT_PD;
         if (st->sticky_status == 0) st->sticky_status = KF_ERR_NOT_PD;
         st->frame_skip_fuse = 1;
@@ -484,20 +553,46 @@ __global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, c
 // and the partial rows are double-buffered by launch parity, so no workgroup
 // reads what another workgroup of the same launch writes.
 
-// Applies the pending solve of `c` (thread 0; tot/count already summed).
-__device__ void apply_pending(IcpCarry& c, const double* tot, double count, PoseState* st, bool record,
-                              float stop_angle, float stop_dist)
+// Applies the solve of the pending iteration of `c` (thread 0): rc and xd from
+// chol6_wave, tot/count the sums it was solved from.
+__device__ void apply_solution(IcpCarry& c, int rc, const double* xd, const double* tot, double count,
+                               PoseState* st, bool record, float stop_angle, float stop_dist)
 {
     if (!c.pending || c.status != 0) return;
     float x[6] = {0, 0, 0, 0, 0, 0};
-    int rc = solve_update(tot, c.R, c.t, &c.level_done[c.pending_level], stop_angle, stop_dist, x);
-    if (rc) c.status = rc;
-    else c.iterations += 1;
+    bool done = false;
+    if (rc) rc = KF_ERR_NOT_PD;
+    else rc = pose_step(xd, c.R, c.t, stop_angle, stop_dist, x, &done);
+    if (rc) {
+        c.status = rc;
+    } else {
+        c.iterations += 1;
+        // c.level_done[c.pending_level] = 1 without a dynamic index (keeps c in registers)
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            if (done && l == c.pending_level) c.level_done[l] = 1;
+    }
     if (record) {
         for (int q = 0; q < kTerms; ++q) st->last_sums[q] = tot[q];
         st->last_inliers = (long long)count;
         for (int q = 0; q < 6; ++q) st->last_x[q] = x[q];
     }
+}
+
+// Wave 0 solves the pending iteration of cin (if any) from tot; every lane of
+// the wave gets rc and xd.
+__device__ __forceinline__ int solve_pending(const IcpCarry* cin, const double* tot, double* xd)
+{
+    if (!(cin->pending && cin->status == 0)) return 0;
+    return chol6_wave(tot, xd);
+}
+
+__device__ __forceinline__ int level_flag(const int* v, int level)
+{
+    int r = 0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) r = (l == level) ? v[l] : r;
+    return r;
 }
 
 // Phase B of a launch for one session: finish the previous iteration from
@@ -511,10 +606,13 @@ __device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* co
                                 double* tot, double* count)
 {
     if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        double xd[6];
+        const int rc = solve_pending(cin, tot, xd);
+        if (threadIdx.x == 0) {
         IcpCarry c = *cin;
-        apply_pending(c, tot, *count, st, first, stop_angle, stop_dist);
-        const int go = (c.status == 0 && !c.level_done[level]) ? 1 : 0;
+        apply_solution(c, rc, xd, tot, *count, st, first, stop_angle, stop_dist);
+        const int go = (c.status == 0 && !level_flag(c.level_done, level)) ? 1 : 0;
         c.pending = go;
         c.pending_level = level;
         c.pending_blocks = blocks;
@@ -524,6 +622,7 @@ __device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* co
         for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
         *run = go;
         if (first) *cout = c;
+        }
     }
     __syncthreads();
 }
@@ -587,9 +686,12 @@ __device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin
                             double* tot, double* count)
 {
     if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
+    if (threadIdx.x >= 64) return;
+    double xd[6];
+    const int rc = solve_pending(cin, tot, xd);
     if (threadIdx.x != 0) return;
     IcpCarry c = *cin;
-    apply_pending(c, tot, *count, st, true, stop_angle, stop_dist);
+    apply_solution(c, rc, xd, tot, *count, st, true, stop_angle, stop_dist);
     for (int q = 0; q < 9; ++q) st->R[q] = c.R[q];
     for (int q = 0; q < 3; ++q) st->t[q] = c.t[q];
     for (int l = 0; l < 4; ++l) st->level_done[l] = c.level_done[l];

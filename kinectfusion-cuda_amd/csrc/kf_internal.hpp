@@ -79,6 +79,8 @@ struct kf_volume {
     int ns = 0;                             // super-bricks per axis (ceil(nb / 8))
     int* brick_list = nullptr;              // visible bricks of the current fuse
     unsigned int* list_count = nullptr;     // list lengths, double-buffered by launch parity
+    float* dmax = nullptr;                  // max valid depth per 16x16 and per 64x64 pixel tile
+    size_t dmax_cap = 0;                    // floats allocated for dmax
     int64_t fuse_launches = 0;
     size_t bytes = 0;
 };

@@ -22,6 +22,7 @@ int launch_preprocess(const uint16_t* raw, int w, int h, float scale, int unmirr
 struct BatchHeader {
     int n_pre, pre_tiles, pyr_tiles;  // preprocess jobs, L0 16x16 tiles, L2 16x16 tiles
     int n_ray, ray_tiles;             // raycast jobs (session x level), 8x8 tiles
+    int n_ray_f32, ray_tiles_f32;     // the FP32-volume jobs come first: their count and tiles
     int n_icp, icp_blocks[3];         // ICP sessions, blocks per level
     int n_fuse;                       // fuse jobs
     unsigned fuse_bricks;             // bricks over all fuse jobs
@@ -36,6 +37,7 @@ struct PreJob {
     const uint16_t* raw;
     float* depth_m;
     float* d0;         // bilateral-filtered level 0
+    float* dmax;       // tile-max maps of depth_m (FuseArgs::dmax layout)
     float4* v[3];      // camera-space vertex maps
     int w, h;
     float scale;
@@ -64,7 +66,16 @@ struct FuseArgs {
     int ns;
     int cull;     // 1 when K is a plain pinhole matrix (culling planes valid)
     int format;   // kf_voxel_format
+    // occlusion test: max valid depth per 16x16 tile (tx16 x ty16) followed by
+    // the max per 64x64 tile (tx64 x ty64); null = frustum test only
+    const float* dmax;
+    int tx16, ty16, tx64, ty64;
 };
+// Floats of the tile-max maps of a w x h depth map.
+inline size_t depth_max_floats(int w, int h)
+{
+    return (size_t)((w + 15) / 16) * ((h + 15) / 16) + (size_t)((w + 63) / 64) * ((h + 63) / 64);
+}
 struct FuseJob {
     FuseArgs a;       // a.dyn set: the session's committed pose
     unsigned brick0;  // first global brick index of this job
@@ -75,9 +86,12 @@ int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_
                       hipStream_t s);
 // inv_pose: by-value world->camera matrix; dyn_state (may be null) overrides it
 // with PoseState::inv_pose and skips the launch's work when the frame failed.
+// list_events (optional): two events recorded around the brick-list kernel alone.
 int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
-                const PoseState* dyn_state, hipStream_t s);
+                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events);
 int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s);
+// kf_volume_create without the host wait: initialisation is enqueued on `s`.
+int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, kf_volume** out);
 
 // ---- raycast (raycast.hip)
 struct RaycastVolume {

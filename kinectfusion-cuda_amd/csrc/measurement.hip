@@ -120,11 +120,13 @@ __global__ __launch_bounds__(256) void vertex_kernel(const float* __restrict__ d
 
 // ---- many sessions per launch (batch ticks), persistent over the tick's tiles.
 // Pass 1, per level-0 16x16 tile: raw codes -> metres in LDS (the fuse input is
-// written back), bilateral filter, level-0 vertex map.  Also clears the tick's
+// written back, with the tile's max valid depth), bilateral filter, level-0
+// vertex map.  Also clears the tick's
 // fuse-list counter, which the fuse passes of this tick use after it.
 __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
 {
     __shared__ BilateralTile tile;
+    __shared__ float wave_max[4];
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hd->fuse_count = 0u;
     const int tiles = hd->pre_tiles, njobs = hd->n_pre;
     const int t = threadIdx.y * kTile + threadIdx.x;
@@ -142,24 +144,42 @@ __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* h
         }
         __syncthreads();
         const int i = i0 + 3 + threadIdx.x, j = j0 + 3 + threadIdx.y;
+        float dv = 0.0f;  // this pixel's contribution to the tile's max valid depth
         if (i < J.w && j < J.h) {
             const int64_t q = (int64_t)j * J.w + i;
-            J.depth_m[q] = tile[threadIdx.y + 3][threadIdx.x + 3];
+            const float dm = tile[threadIdx.y + 3][threadIdx.x + 3];
+            J.depth_m[q] = dm;
+            dv = is_depth_valid(dm) ? dm : 0.0f;
             const float d = bilateral_at(tile, threadIdx.x, threadIdx.y);
             J.d0[q] = d;
             J.v[0][q] = vertex_at(d, i, j, J.Kinv[0]);
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dv = fmaxf(dv, __shfl_xor(dv, off, 64));
+        if ((t & 63) == 0) wave_max[t >> 6] = dv;
+        __syncthreads();
+        if (t == 0) J.dmax[lt] = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));
     }
 }
 
 // Pass 2, one thread per level-2 pixel: its 4x4 level-0 block gives 4 level-1
 // depths (downSample) and those the level-2 depth; vertex maps of both levels.
+// A workgroup covers a 64x64 level-0 region and also writes its max valid depth.
 __global__ __launch_bounds__(256) void pyramid_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
 {
     const int tiles = hd->pyr_tiles, njobs = hd->n_pre;
     for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
         const PreJob& J = jobs[job_of(jobs, njobs, tl, [](const PreJob& x) { return x.ptile0; })];
         const int lt = tl - J.ptile0;
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            // the 64x64 region of this tile: max of its 16x16 tiles' maxima
+            const int tx16 = J.tiles_x, ty16 = (J.h + 15) / 16;
+            const int rx = lt % J.ptiles_x, ry = lt / J.ptiles_x;
+            float m = 0.0f;
+            for (int qy = ry * 4; qy < min(ry * 4 + 4, ty16); ++qy)
+                for (int qx = rx * 4; qx < min(rx * 4 + 4, tx16); ++qx) m = fmaxf(m, J.dmax[qy * tx16 + qx]);
+            J.dmax[tx16 * ty16 + lt] = m;
+        }
         const int i2 = (lt % J.ptiles_x) * kTile + threadIdx.x, j2 = (lt / J.ptiles_x) * kTile + threadIdx.y;
         const int w1 = J.w >> 1, w2 = J.w >> 2;
         if (i2 >= w2 || j2 >= (J.h >> 2)) continue;

@@ -25,7 +25,8 @@ struct kf_pipeline {
     float K[3][9], Ki[3][9];
     int levels = 3;
     int64_t frame = 0;
-    // timing: 5 events per frame (start, after preprocess, after raycast, after ICP, after fuse)
+    // timing: 7 events per frame (start, after preprocess, after raycast, after ICP, after fuse,
+    // and the pair around the fuse's brick-list kernel)
     bool timing = false;
     std::vector<hipEvent_t> pending;
     float stage_ms[KF_STAGE_COUNT] = {0, 0, 0, 0};
@@ -45,13 +46,14 @@ int alloc(void** p, size_t bytes)
 // Folds the recorded events of finished frames into the stage totals.
 int drain_events(kf_pipeline* p)
 {
-    for (size_t f = 0; f + 5 <= p->pending.size(); f += 5) {
+    for (size_t f = 0; f + 7 <= p->pending.size(); f += 7) {
         hipEvent_t* e = &p->pending[f];
         KF_HIP(hipEventSynchronize(e[4]));
-        float ms[4];
+        float ms[4], list_ms = 0.0f;
         for (int q = 0; q < 4; ++q) KF_HIP(hipEventElapsedTime(&ms[q], e[q], e[q + 1]));
+        KF_HIP(hipEventElapsedTime(&list_ms, e[5], e[6]));
         for (int q = 0; q < 4; ++q) p->stage_ms[q] += ms[q];
-        p->fuse_ms += ms[3];
+        p->fuse_ms += list_ms;
         p->fuse_launches += 1;
         p->timed_frames += 1;
     }
@@ -150,7 +152,7 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
     if (!p || !raw) return kf::fail(KF_ERR_ARG, "kf_pipeline_frame: null argument");
     const int w = p->cfg.frame_width, h = p->cfg.frame_height;
     hipStream_t s = p->s;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (p->timing) {
         for (auto& e : ev) KF_HIP(hipEventCreate(&e));
         KF_HIP(hipEventRecord(ev[0], s));
@@ -186,7 +188,8 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
         if (p->timing) KF_HIP(hipEventRecord(ev[3], s));
     }
     // fuseCurrentDepthToTSDF (kinect_fusion.cpp:179-183): unfiltered metres at the new pose
-    if ((st = kf::launch_fuse(p->depth_m, w, h, p->vol, p->K[0], nullptr, state, s))) return st;
+    if ((st = kf::launch_fuse(p->depth_m, w, h, p->vol, p->K[0], nullptr, state, s, p->timing ? ev + 5 : nullptr)))
+        return st;
     if (p->timing) {
         KF_HIP(hipEventRecord(ev[4], s));
         for (auto e : ev) p->pending.push_back(e);

@@ -108,14 +108,18 @@ __device__ __forceinline__ bool skippable(const RaycastVolume& a, const Stencil&
     return b0 == b1 && a.brick_min[b0] >= kSkipBound;
 }
 
-// Conservative skip test on approximate texel coordinates (xa, ya, za): true
-// only if the exact stencil lies inside one super-brick (or one brick) whose
-// lower bound of F is >= kSkipBound.  The last super-brick's bound is cached
-// in (cs, cb), so a march through free space reads no memory at all.
-__device__ __forceinline__ bool fast_skip(const RaycastVolume& a, float xa, float ya, float za, int& cs, float& cb)
+// Conservative skip test on approximate texel coordinates (xa, ya, za).
+// Returns 1 when the exact stencil lies inside one super-brick (or one brick)
+// whose lower bound of F is >= kSkipBound, 0 when it lies inside one brick
+// whose bound is below it (so skippable() would say no), -1 when the margins
+// leave it open.  The last super-brick's and brick's bounds are cached in
+// (cs, cb) and (ck, cbk), so a march through free space, or along a brick, reads
+// no memory.
+__device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float ya, float za, int& cs, float& cb,
+                                         int& ck, float& cbk)
 {
     const float hi = (float)a.n - 1.0f - kMargin;
-    if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return false;
+    if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return -1;
     const int xi = (int)xa, yi = (int)ya, zi = (int)za;
     const int sx = xi >> 6, sy = yi >> 6, sz = zi >> 6;
     const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
@@ -126,19 +130,56 @@ __device__ __forceinline__ bool fast_skip(const RaycastVolume& a, float xa, floa
             cs = sidx;
             cb = order_value(a.super_key[sidx]);
         }
-        if (cb >= kSkipBound) return true;
+        if (cb >= kSkipBound) return 1;
     }
     const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
     const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
     const float bhi = (float)(kBrick - 1) - kMargin;
-    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi)
-        return a.brick_min[(bz * a.nb + by) * a.nb + bx] >= kSkipBound;
-    return false;
+    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi) {
+        const int bidx = (bz * a.nb + by) * a.nb + bx;
+        if (bidx != ck) {
+            ck = bidx;
+            cbk = a.brick_min[bidx];
+        }
+        return cbk >= kSkipBound ? 1 : 0;
+    }
+    return -1;
+}
+
+// The corner values of the last trilinear cell a march read: consecutive small
+// steps (0.1 mue) often stay in one cell, and then need no loads.
+struct CellCache {
+    int ki = -1, kj = -1, kk = -1;  // (i0 | i1 << 16) per axis
+    float c[8];
+};
+
+template <int FMT>
+__device__ __forceinline__ float trilinear_cached(const void* F, int nb, const Stencil& t, CellCache& cc)
+{
+    const int ki = t.i0 | (t.i1 << 16), kj = t.j0 | (t.j1 << 16), kk = t.k0 | (t.k1 << 16);
+    if (ki != cc.ki || kj != cc.kj || kk != cc.kk) {
+        cc.ki = ki;
+        cc.kj = kj;
+        cc.kk = kk;
+        cc.c[0] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k0));
+        cc.c[1] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k0));
+        cc.c[2] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k0));
+        cc.c[3] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k0));
+        cc.c[4] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k1));
+        cc.c[5] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k1));
+        cc.c[6] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k1));
+        cc.c[7] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k1));
+    }
+    const float a = t.a, b = t.b, c = t.c;
+    float e00 = (1.0f - a) * cc.c[0] + a * cc.c[1], e10 = (1.0f - a) * cc.c[2] + a * cc.c[3];
+    float e01 = (1.0f - a) * cc.c[4] + a * cc.c[5], e11 = (1.0f - a) * cc.c[6] + a * cc.c[7];
+    float g0 = (1.0f - b) * e00 + b * e10, g1 = (1.0f - b) * e01 + b * e11;
+    return (1.0f - c) * g0 + c * g1;
 }
 
 // raycast.cu:44-145 for pixel (i, j) of one level map.
 template <int FMT>
-__device__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i, int j)
+__device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i, int j)
 {
     float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     float M[9];
@@ -173,25 +214,28 @@ __device__ void raycast_pixel(const RaycastVolume& a, const float* pose, const R
     float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f;
     bool prev_skipped = false;
     const float invw = 1.0f / width, nf = (float)a.n;
-    int cs = -1;
-    float cb = 0.0f;
+    int cs = -1, ck = -1;
+    float cb = 0.0f, cbk = 0.0f;
+    CellCache cc;
     int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
     for (float t = nearv; t < farv; t = t + step) {
         if (++guard > kMaxMarchSteps) break;
         v3 p = add3(o, mul3s(dir, t));
-        if (a.brick_min && fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
-                                     (p.z * invw + 0.5f) * nf - 0.5f, cs, cb)) {
+        const int fs = a.brick_min ? fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
+                                               (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
+                                   : 0;
+        if (fs == 1) {
             prev_skipped = true;
             prev_t = t;
             continue;
         }
         const Stencil st = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
-        if (skippable(a, st)) {
+        if (fs < 0 && skippable(a, st)) {
             prev_skipped = true;
             prev_t = t;
             continue;
         }
-        float f = trilinear<FMT>(a.f, a.nb, st);
+        float f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
         if (f < 0.0f) {
             if (t != nearv) {
                 if (step == large) {
@@ -220,13 +264,21 @@ __device__ void raycast_pixel(const RaycastVolume& a, const float* pose, const R
         // raycast.cu:29-42
         float r = mue * 0.5f / width;
         float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
-        float fx0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u + r, v, s);
-        float fy0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v + r, s);
-        float fz0 = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s + r);
-        float fx = sample_tsdf<FMT>(a.f, a.n, a.nb, u - r, v, s);
-        float fy = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v - r, s);
-        float fz = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s - r);
-        v3 g = div3s(mk3(fx - fx0, fy - fy0, fz - fz0), r);
+        // one axis at a time (16 loads in flight, not 48): fewer live registers,
+        // more waves per SIMD for the march.  x + 0 == x, so the samples are the
+        // six of raycast.cu:33-38.
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll 1
+        for (int ax = 0; ax < 3; ++ax) {
+            const float du = ax == 0 ? r : 0.0f, dv = ax == 1 ? r : 0.0f, ds = ax == 2 ? r : 0.0f;
+            const float fp = sample_tsdf<FMT>(a.f, a.n, a.nb, u + du, v + dv, s + ds);
+            const float fm = sample_tsdf<FMT>(a.f, a.n, a.nb, u - du, v - dv, s - ds);
+            const float d = fm - fp;
+            gx = ax == 0 ? d : gx;
+            gy = ax == 1 ? d : gy;
+            gz = ax == 2 ? d : gz;
+        }
+        v3 g = div3s(mk3(gx, gy, gz), r);
         float gn = len3(g);
         if (is_depth_valid(gn)) {
             v3 nn = div3s(g, gn);
@@ -249,7 +301,7 @@ __device__ __forceinline__ int xcd_tile(int tiles)
 }
 
 template <int FMT>
-__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
+__global__ __launch_bounds__(64, 8) void raycast_kernel(RaycastArgs a)
 {
     const int tile = xcd_tile(a.tiles);
     if (tile >= a.tiles) return;
@@ -263,28 +315,63 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
     raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j);
 }
 
-// Many sessions and levels in one persistent launch: jobs[k] covers tiles
-// [jobs[k].L.tile0, jobs[k+1].L.tile0) of the tick; workgroup ranks take
-// contiguous tile bands (XCD-contiguous) and stride through them.
-__global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+// Field-wise copy of a job read through the constant address space.
+template <class CJ>
+__device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, RaycastLevel& L)
 {
-    const int tiles = hd->ray_tiles, njobs = hd->n_ray;
+    vol.f = j.vol.f;
+    vol.brick_min = j.vol.brick_min;
+    vol.super_key = j.vol.super_key;
+    vol.n = j.vol.n;
+    vol.nb = j.vol.nb;
+    vol.ns = j.vol.ns;
+    vol.format = j.vol.format;
+    vol.width = j.vol.width;
+    vol.mue = j.vol.mue;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) L.Kinv[q] = j.L.Kinv[q];
+    L.vout = j.L.vout;
+    L.nout = j.L.nout;
+    L.w = j.L.w;
+    L.h = j.L.h;
+    L.tiles_x = j.L.tiles_x;
+    L.tile0 = j.L.tile0;
+}
+
+// Many sessions and levels per persistent launch, one launch per voxel format:
+// the jobs of format FMT are jobs[j0, j1), covering tiles [t0, t1) of the tick
+// (jobs[k] covers [jobs[k].L.tile0, jobs[k+1].L.tile0)); workgroup ranks take
+// contiguous tile bands (XCD-contiguous).
+template <int FMT>
+__global__ __launch_bounds__(64, 8) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+{
+    const int nf = hd->n_ray_f32, tf = hd->ray_tiles_f32;
+    const int j0 = FMT == KF_VOXEL_F32 ? 0 : nf, j1 = FMT == KF_VOXEL_F32 ? nf : hd->n_ray;
+    const int t0 = FMT == KF_VOXEL_F32 ? 0 : tf, t1 = FMT == KF_VOXEL_F32 ? tf : hd->ray_tiles;
     const int G = (int)gridDim.x;
     const int rank = xcd_rank((int)blockIdx.x, G);
-    const int chunk = (tiles + G - 1) / G;
-    const int end = min(tiles, (rank + 1) * chunk);
-    int t = rank * chunk;
+    const int chunk = (t1 - t0 + G - 1) / G;
+    const int end = min(t1, t0 + (rank + 1) * chunk);
+    int t = t0 + rank * chunk;
     if (t >= end) return;
-    int jb = job_of(jobs, njobs, t, [](const RaycastJob& x) { return x.L.tile0; });
+    // The table is read through the constant address space: it is never written
+    // while the kernel runs, so its fields load with scalar loads into scalar
+    // registers even after the kernel's stores (the vertex and normal maps).
+    typedef __attribute__((address_space(4))) const RaycastJob ConstJob;
+    ConstJob* cj = (ConstJob*)jobs;
+    int jb = j0 + job_of(jobs + j0, j1 - j0, t, [](const RaycastJob& x) { return x.L.tile0; });
     for (; t < end; ++t) {
-        while (jb + 1 < njobs && jobs[jb + 1].L.tile0 <= t) ++jb;
-        const RaycastJob& J = jobs[jb];
-        const int lt = t - J.L.tile0;
-        const int i = (lt % J.L.tiles_x) * 8 + (threadIdx.x & 7);
-        const int j = (lt / J.L.tiles_x) * 8 + (threadIdx.x >> 3);
-        if (i >= J.L.w || j >= J.L.h) continue;
-        if (J.vol.format == KF_VOXEL_F32) raycast_pixel<KF_VOXEL_F32>(J.vol, J.pose, J.L, i, j);
-        else raycast_pixel<KF_VOXEL_SNORM16_U8>(J.vol, J.pose, J.L, i, j);
+        while (jb + 1 < j1 && cj[jb + 1].L.tile0 <= t) ++jb;
+        jb = __builtin_amdgcn_readfirstlane(jb);
+        RaycastVolume vol;
+        RaycastLevel L;
+        copy_job(cj[jb], vol, L);
+        const float* pose = cj[jb].pose;
+        const int lt = t - L.tile0;
+        const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
+        const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
+        if (i >= L.w || j >= L.h) continue;
+        raycast_pixel<FMT>(vol, pose, L, i, j);
     }
 }
 
@@ -307,7 +394,9 @@ RaycastVolume raycast_volume(const kf_volume* v)
 
 int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s)
 {
-    raycast_batch_kernel<<<blocks, 64, 0, s>>>(hd, jobs);
+    raycast_batch_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("raycast_batch_kernel");
+    raycast_batch_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
     return KF_OK;
 }

@@ -42,7 +42,66 @@ __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip
     if (((W - px) * ez - fx * ex) < -r * sqrtf(fx * fx + (W - px) * (W - px))) return false;
     if ((fy * ey + py * ez) < -r * sqrtf(fy * fy + py * py)) return false;
     if (((H - py) * ez - fy * ey) < -r * sqrtf(fy * fy + (H - py) * (H - py))) return false;
-    return true;
+    // Occlusion: a voxel is updated only if d - z >= -mue for the valid depth d
+    // of its pixel.  With the sphere beyond the near plane its pixels lie in the
+    // box below (x/z and y/z at the corners, one pixel of margin); if even the
+    // largest valid depth there is more than mue (+1 mm for rounding) in front
+    // of the sphere's nearest point, no voxel of the brick is updated.
+    const float zlo = ez - r;
+    if (!a.dmax || !(zlo > kMinDistance)) return true;
+    const float zhi = ez + r, xa = ex - r, xb = ex + r, ya = ey - r, yb = ey + r;
+    const float u0 = fx * fminf(xa / zlo, xa / zhi) + px - 1.0f, u1 = fx * fmaxf(xb / zlo, xb / zhi) + px + 1.0f;
+    const float v0 = fy * fminf(ya / zlo, ya / zhi) + py - 1.0f, v1 = fy * fmaxf(yb / zlo, yb / zhi) + py + 1.0f;
+    const int i0 = (int)fmaxf(u0, 0.0f), i1 = (int)fminf(u1, W - 1.0f);
+    const int j0 = (int)fmaxf(v0, 0.0f), j1 = (int)fminf(v1, H - 1.0f);
+    const float* m;
+    int shift, tx;
+    if (i1 - i0 < 48 && j1 - j0 < 48) {
+        m = a.dmax;
+        shift = 4;
+        tx = a.tx16;
+    } else if (i1 - i0 < 192 && j1 - j0 < 192) {
+        m = a.dmax + a.tx16 * a.ty16;
+        shift = 6;
+        tx = a.tx64;
+    } else {
+        return true;
+    }
+    float dm = 0.0f;
+    for (int tj = j0 >> shift; tj <= (j1 >> shift); ++tj)
+        for (int ti = i0 >> shift; ti <= (i1 >> shift); ++ti) dm = fmaxf(dm, m[tj * tx + ti]);
+    return dm + a.mue + 1e-3f >= zlo;
+}
+
+// Tile-max maps of a depth map (FuseArgs::dmax): one workgroup per 64x64
+// pixel region; lane group t/16 takes one 16x16 tile, each lane a 4x4 block.
+// Invalid depths (isDepthValid false) count as 0.
+__global__ __launch_bounds__(256) void depth_max_kernel(const float* __restrict__ depth, int w, int h, float* dmax,
+                                                        int tx16, int ty16, int tx64)
+{
+    __shared__ float part[16];
+    const int q = threadIdx.x >> 4, sub = threadIdx.x & 15;
+    const int tx = blockIdx.x * 4 + (q & 3), ty = blockIdx.y * 4 + (q >> 2);
+    const int x0 = tx * 16 + (sub & 3) * 4, y0 = ty * 16 + (sub >> 2) * 4;
+    float m = 0.0f;
+    for (int y = y0; y < y0 + 4; ++y)
+        for (int x = x0; x < x0 + 4; ++x)
+            if (x < w && y < h) {
+                const float d = depth[(int64_t)y * w + x];
+                if (is_depth_valid(d)) m = fmaxf(m, d);
+            }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (sub == 0) {
+        part[q] = m;
+        if (tx < tx16 && ty < ty16) dmax[ty * tx16 + tx] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int k = 0; k < 16; ++k) r = fmaxf(r, part[k]);
+        dmax[tx16 * ty16 + blockIdx.y * tx64 + blockIdx.x] = r;
+    }
 }
 
 // tsdf.cu:20-75 for the 8 voxels of one brick column (lane = (j, i) of the
@@ -207,9 +266,19 @@ __global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, 
                                                                     const unsigned int* count)
 {
     if (a.dyn && a.dyn->frame_skip_fuse) return;
-    __shared__ float ip[16];
     __shared__ int wave_cnt[kFuseWaves];
-    load_inv_pose(a, ip);
+    // the pose is uniform: scalar loads (constant address space: no kernel
+    // writes the PoseState while this one runs)
+    typedef __attribute__((address_space(4))) const PoseState ConstState;
+    float ip[16];
+    if (a.dyn) {
+        ConstState* cs = (ConstState*)a.dyn;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ip[q] = cs->inv_pose[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ip[q] = a.inv_pose[q];
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned int total = *count;
     const unsigned int waves = gridDim.x * kFuseWaves;
@@ -266,26 +335,64 @@ __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(cons
     }
 }
 
+// Field-wise copy of a job's arguments read through the constant address
+// space (scalar loads; the inverse pose comes from the session's PoseState).
+template <class CJ>
+__device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
+{
+    a.depth = j.a.depth;
+    a.w = j.a.w;
+    a.h = j.a.h;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) a.K[q] = j.a.K[q];
+    a.dyn = j.a.dyn;
+    a.n = j.a.n;
+    a.nb = j.a.nb;
+    a.half = j.a.half;
+    a.res = j.a.res;
+    a.mue = j.a.mue;
+    a.zk = j.a.zk;
+    a.f = j.a.f;
+    a.wt = j.a.wt;
+    a.brick_min = j.a.brick_min;
+    a.super_key = j.a.super_key;
+    a.ns = j.a.ns;
+    a.format = j.a.format;
+}
+
 __global__ __launch_bounds__(64 * kFuseWaves) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
 {
     __shared__ int wave_cnt[kFuseWaves];
+    // the tick's table is never written while the kernel runs: read it with
+    // scalar loads through the constant address space
+    typedef __attribute__((address_space(4))) const FuseJob ConstJob;
+    typedef __attribute__((address_space(4))) const BatchHeader ConstHeader;
+    ConstJob* cj = (ConstJob*)jobs;
+    ConstHeader* ch = (ConstHeader*)hd;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned int total = *hd->fuse_count;
+    const unsigned int total = *ch->fuse_count;
     const unsigned int waves = gridDim.x * kFuseWaves;
-    const int njobs = hd->n_fuse;
-    int cnt = 0, cur = -1;
-    float ip[16];
+    const int njobs = ch->n_fuse;
+    const unsigned* list = ch->fuse_list;
+    int cnt = 0;
     for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves) {
-        const unsigned g = hd->fuse_list[item];
-        const int j = job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; });
-        const FuseJob& J = jobs[j];
-        if (j != cur) {
-            cur = j;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) ip[q] = J.a.dyn->inv_pose[q];
+        const unsigned g = __builtin_amdgcn_readfirstlane(list[item]);
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cj[mid].brick0 <= g) lo = mid;
+            else hi = mid - 1;
         }
-        if (J.a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(J.a, ip, g - J.brick0, lane);
-        else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(J.a, ip, g - J.brick0, lane);
+        FuseArgs a;
+        load_fuse_args(cj[lo], a);
+        const int64_t brick = g - cj[lo].brick0;
+        typedef __attribute__((address_space(4))) const PoseState ConstState;
+        ConstState* cs = (ConstState*)a.dyn;
+        float ip[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ip[q] = cs->inv_pose[q];
+        if (a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(a, ip, brick, lane);
+        else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(a, ip, brick, lane);
     }
     add_touched(hd->touched, cnt, wave_cnt);
 }
@@ -379,6 +486,16 @@ __global__ __launch_bounds__(256) void brick_min_kernel(const void* f, float* br
     }
 }
 
+// z of each k slice: one lane, the host loop's repeated additions in order.
+__global__ void z_table_kernel(float* zk, int n, float half, float res)
+{
+    float z = half + (-res);
+    for (int k = 0; k < n; ++k) {
+        z = z + res;
+        zk[k] = z;
+    }
+}
+
 bool is_pinhole(const float* K)
 {
     return K[1] == 0.0f && K[2] == 0.0f && K[3] == 0.0f && K[5] == 0.0f && K[8] == 1.0f && K[0] > 0.0f &&
@@ -459,6 +576,11 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
     a.ns = v->ns;
     a.cull = is_pinhole(K) ? 1 : 0;
     a.format = v->format;
+    a.dmax = nullptr;
+    a.tx16 = (w + 15) / 16;
+    a.ty16 = (h + 15) / 16;
+    a.tx64 = (w + 63) / 64;
+    a.ty64 = (h + 63) / 64;
     return a;
 }
 
@@ -473,9 +595,23 @@ int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_
 }
 
 int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
-                const PoseState* dyn_state, hipStream_t s)
+                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events)
 {
+    const size_t need = depth_max_floats(w, h);
+    if (v->dmax_cap < need) {
+        // grown outside the stream order: wait for work that may still read the old map
+        KF_HIP(hipStreamSynchronize(s));
+        (void)hipFree(v->dmax);
+        v->dmax = nullptr;
+        v->dmax_cap = 0;
+        KF_HIP(hipMalloc(&v->dmax, sizeof(float) * need));
+        v->dmax_cap = need;
+    }
     FuseArgs a = fuse_args(depth, w, h, v, K, inv_pose, dyn_state);
+    a.dmax = v->dmax;
+    depth_max_kernel<<<dim3((unsigned)a.tx64, (unsigned)a.ty64), 256, 0, s>>>(depth, w, h, v->dmax, a.tx16, a.ty16,
+                                                                            a.tx64);
+    KF_LAUNCHED("depth_max_kernel");
     const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
     const int parity = (int)(v->fuse_launches & 1);
     fuse_cull_kernel<<<(unsigned)((nbricks + kCullThreads - 1) / kCullThreads), kCullThreads, 0, s>>>(
@@ -483,12 +619,14 @@ int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, 
     KF_LAUNCHED("fuse_cull_kernel");
     // 256 CUs x 8 resident workgroups of 4 waves; fewer when the volume is small
     const unsigned blocks = (unsigned)std::min<int64_t>((nbricks + kFuseWaves - 1) / kFuseWaves, kFuseBlocks);
+    if (list_events) KF_HIP(hipEventRecord(list_events[0], s));
     if (v->format == KF_VOXEL_F32)
         fuse_list_kernel<KF_VOXEL_F32><<<blocks, 64 * kFuseWaves, 0, s>>>(a, v->brick_list, v->list_count + parity);
     else
         fuse_list_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64 * kFuseWaves, 0, s>>>(a, v->brick_list,
                                                                                  v->list_count + parity);
     KF_LAUNCHED("fuse_list_kernel");
+    if (list_events) KF_HIP(hipEventRecord(list_events[1], s));
     v->fuse_launches += 1;
     return KF_OK;
 }
@@ -509,22 +647,18 @@ int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s)
     return KF_OK;
 }
 
-}  // namespace kf
-
-// ------------------------------------------------------------------ C ABI
-extern "C" {
-
-int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** out)
+// Allocates a volume and enqueues its initialisation on `s` (no host wait):
+// the z table by repeated addition and every voxel at (2, 0).
+int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, kf_volume** out)
 {
-    if (!out) return kf::fail(KF_ERR_ARG, "kf_volume_create: null out");
+    if (!out) return fail(KF_ERR_ARG, "kf_volume_create: null out");
     *out = nullptr;
-    if (n < 8 || n % 8 != 0) return kf::fail(KF_ERR_ARG, "kf_volume_create: n must be a positive multiple of 8");
-    if (!(width_m > 0.0f) || !(width_m < 1e4f))
-        return kf::fail(KF_ERR_ARG, "kf_volume_create: width must be in (0, 1e4) m");
+    if (n < 8 || n % 8 != 0) return fail(KF_ERR_ARG, "kf_volume_create: n must be a positive multiple of 8");
+    if (!(width_m > 0.0f) || !(width_m < 1e4f)) return fail(KF_ERR_ARG, "kf_volume_create: width must be in (0, 1e4) m");
     // the raycast marches at >= 0.1 * mue; 1e-4 m keeps every march finite and short
-    if (!(mue_m >= 1e-4f)) return kf::fail(KF_ERR_ARG, "kf_volume_create: mue must be >= 1e-4 m");
+    if (!(mue_m >= 1e-4f)) return fail(KF_ERR_ARG, "kf_volume_create: mue must be >= 1e-4 m");
     if (format != KF_VOXEL_F32 && format != KF_VOXEL_SNORM16_U8)
-        return kf::fail(KF_ERR_ARG, "kf_volume_create: unknown voxel format");
+        return fail(KF_ERR_ARG, "kf_volume_create: unknown voxel format");
     kf_volume* v = new kf_volume();
     v->n = n;
     v->nb = n / 8;
@@ -539,35 +673,48 @@ int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** 
     hipError_t e;
     if ((e = hipMalloc(&v->f, (size_t)count * fb)) != hipSuccess || (e = hipMalloc(&v->w, (size_t)count * wb)) != hipSuccess ||
         (e = hipMalloc(&v->zk, sizeof(float) * n)) != hipSuccess ||
-        (e = hipMalloc(&v->touched, sizeof(unsigned long long) * kf::kFuseBlocks)) != hipSuccess ||
-        (e = hipMemset(v->touched, 0, sizeof(unsigned long long) * kf::kFuseBlocks)) != hipSuccess ||
+        (e = hipMalloc(&v->touched, sizeof(unsigned long long) * kFuseBlocks)) != hipSuccess ||
+        (e = hipMemsetAsync(v->touched, 0, sizeof(unsigned long long) * kFuseBlocks, s)) != hipSuccess ||
         (e = hipMalloc(&v->brick_list, sizeof(int) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
         (e = hipMalloc(&v->brick_min, sizeof(float) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
         (e = hipMalloc(&v->super_key, sizeof(int) * (size_t)v->ns * v->ns * v->ns)) != hipSuccess ||
         (e = hipMalloc(&v->list_count, 2 * sizeof(unsigned int))) != hipSuccess ||
-        (e = hipMemset(v->list_count, 0, 2 * sizeof(unsigned int))) != hipSuccess) {
+        (e = hipMemsetAsync(v->list_count, 0, 2 * sizeof(unsigned int), s)) != hipSuccess) {
         kf_volume_destroy(v);
-        return kf::fail_hip(e, "kf_volume_create");
+        return fail_hip(e, "kf_volume_create");
     }
     // tsdf.cu:27-31: z starts at -width/2 - res and advances by repeated addition
-    std::vector<float> zk(n);
-    float z = -width_m * 0.5f + (-v->res);
-    for (int k = 0; k < n; ++k) {
-        z = z + v->res;
-        zk[k] = z;
-    }
-    if ((e = hipMemcpy(v->zk, zk.data(), sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) {
+    z_table_kernel<<<1, 1, 0, s>>>(v->zk, n, -width_m * 0.5f, v->res);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
         kf_volume_destroy(v);
-        return kf::fail_hip(e, "kf_volume_create");
+        return fail_hip(e, "z_table_kernel");
     }
     // Voxel{} = (2, 0) (voxel_grid.h:10-16), as KinectFusion's constructor initialises it
-    int st = kf::launch_initialize(v, 2.0f, 0.0f, nullptr);
-    if (st == KF_OK && (e = hipDeviceSynchronize()) != hipSuccess) st = kf::fail_hip(e, "initialize");
+    int st = launch_initialize(v, 2.0f, 0.0f, s);
     if (st != KF_OK) {
         kf_volume_destroy(v);
         return st;
     }
     *out = v;
+    return KF_OK;
+}
+
+}  // namespace kf
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int kf_volume_create(int n, float width_m, float mue_m, int format, kf_volume** out)
+{
+    int st = kf::volume_create(n, width_m, mue_m, format, nullptr, out);
+    if (st) return st;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        kf_volume_destroy(*out);
+        *out = nullptr;
+        return kf::fail_hip(e, "initialize");
+    }
     return KF_OK;
 }
 
@@ -589,6 +736,7 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->brick_min);
     (void)hipFree(v->super_key);
     (void)hipFree(v->list_count);
+    (void)hipFree(v->dmax);
     delete v;
     return KF_OK;
 }
@@ -640,7 +788,7 @@ int kf_fuse(const float* depth_m, int w, int h, kf_volume* v, const kf_sensor* s
     hipStream_t s = kf::stream_of(stream);
     kf::CallTimer t(elapsed_ms, s);
     int st;
-    if ((st = t.start()) || (st = kf::launch_fuse(depth_m, w, h, v, sensor->intr[0], sensor->inv_pose, nullptr, s)))
+    if ((st = t.start()) || (st = kf::launch_fuse(depth_m, w, h, v, sensor->intr[0], sensor->inv_pose, nullptr, s, nullptr)))
         return st;
     return t.finish();
 }

@@ -14,14 +14,15 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import (KF_ERR_ARG, KF_ERR_DIMS, KF_ERR_IO, KF_ERR_NONFINITE, KF_ERR_NOT_PD, KF_OK,
+from ._capi import (KF_ERR_ARG, KF_ERR_BUDGET, KF_ERR_DIMS, KF_ERR_IO, KF_ERR_NONFINITE, KF_ERR_NOT_PD, KF_OK,
                     KF_VOXEL_F32, KF_VOXEL_SNORM16_U8, STAGE_NAMES, kf_batch_config, kf_batch_shape,
-                    kf_icp_config, kf_pipeline_config, kf_session_config, kf_sensor)
+                    kf_icp_config, kf_pipeline_config, kf_poisson_load, kf_sensor, kf_server_config,
+                    kf_server_stats, kf_session_config)
 
 __all__ = [
     "KfError", "lib", "init", "DeviceArray", "GridMap", "DepthMap", "GridMapPyramid", "Sensor",
     "RigidTransform3D", "VoxelGrid", "ICP", "IcpConfig", "LinearLeastSquares", "kernel", "Pipeline",
-    "Batch", "SessionConfig", "session_bytes", "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
+    "Batch", "SessionConfig", "session_bytes", "Server", "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
 ]
 
 
@@ -609,6 +610,123 @@ class Batch:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             lib().kf_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (kf_server_*): HBM-budget admission, per-session frame queues,
+    continuous batching over a Batch, per-frame latency."""
+
+    def __init__(self, max_sessions, hbm_budget_bytes, icp=None, use_graph=True, autotune=False, shape=None,
+                 queue_depth=8):
+        init()
+        c = kf_server_config()
+        c.batch.max_sessions = max_sessions
+        c.batch.icp = icp if icp is not None else IcpConfig()
+        c.batch.use_graph = int(use_graph)
+        c.batch.autotune = int(autotune)
+        if shape:
+            for k, v in shape.items():
+                setattr(c.batch.shape, k, int(v))
+        c.hbm_budget_bytes = int(hbm_budget_bytes)
+        c.queue_depth = queue_depth
+        self.cfg = c
+        h = C.c_void_p()
+        check(lib().kf_server_create(C.byref(c), C.byref(h)), "kf_server_create")
+        self.h = h
+        self.configs = {}
+
+    @staticmethod
+    def now():
+        return lib().kf_server_now()
+
+    def open(self, cfg):
+        t = C.c_int()
+        check(lib().kf_server_open(self.h, C.byref(cfg), C.byref(t)), "kf_server_open")
+        self.configs[t.value] = cfg
+        return t.value
+
+    def close_session(self, ticket):
+        check(lib().kf_server_close(self.h, ticket), "kf_server_close")
+
+    def submit(self, ticket, frame, arrival_s):
+        f = np.ascontiguousarray(frame, np.uint16)
+        return lib().kf_server_submit(self.h, ticket, _p(f), float(arrival_s))
+
+    def poll(self):
+        l = C.c_int()
+        check(lib().kf_server_poll(self.h, C.byref(l)), "kf_server_poll")
+        return bool(l.value)
+
+    def drain(self):
+        check(lib().kf_server_drain(self.h), "kf_server_drain")
+
+    def state(self, ticket):
+        s = C.c_int()
+        check(lib().kf_server_state(self.h, ticket, C.byref(s)), "kf_server_state")
+        return s.value
+
+    def pose(self, ticket):
+        p = np.zeros(16, np.float32)
+        check(lib().kf_server_pose(self.h, ticket, _p(p)), "kf_server_pose")
+        return p
+
+    def volume_download(self, ticket):
+        v = C.c_void_p()
+        check(lib().kf_server_volume(self.h, ticket, C.byref(v)), "kf_server_volume")
+        n = self.configs[ticket].volume_n
+        fw = np.empty(n ** 3 * 2, np.float32)
+        check(lib().kf_volume_download(v, _p(fw)), "kf_volume_download")
+        return fw[0::2].copy(), fw[1::2].copy()
+
+    def stats(self):
+        st = kf_server_stats()
+        check(lib().kf_server_stats_get(self.h, C.byref(st)), "kf_server_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def latencies(self):
+        n = C.c_size_t()
+        check(lib().kf_server_latencies(self.h, None, 0, C.byref(n)), "kf_server_latencies")
+        out = np.zeros(n.value, np.float64)
+        check(lib().kf_server_latencies(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)),
+              "kf_server_latencies")
+        return out
+
+    def run_load(self, configs, banks, frames_per_session, opens_s, rate_hz=30.0, jitter=0.1, seed=7,
+                 keep_open=False):
+        """Drives the loop with a seeded Poisson load (kf_server_run_load); banks[k] is a
+        (frames, h, w) uint16 array replayed ping-pong.  Returns (tickets, stats)."""
+        S = len(configs)
+        cfg_arr = (kf_session_config * S)(*configs)
+        banks = [np.ascontiguousarray(b, np.uint16) for b in banks]
+        bank_ptrs = (C.c_void_p * S)(*[b.ctypes.data for b in banks])
+        blen = (C.c_int * S)(*[b.shape[0] for b in banks])
+        fps = (C.c_int * S)(*[int(x) for x in frames_per_session])
+        opens = (C.c_double * S)(*[float(x) for x in opens_s])
+        L = kf_poisson_load()
+        L.sessions = S
+        L.configs = C.cast(cfg_arr, C.c_void_p)
+        L.banks = C.cast(bank_ptrs, C.c_void_p)
+        L.bank_len = C.cast(blen, C.c_void_p)
+        L.frames_per_session = C.cast(fps, C.c_void_p)
+        L.opens_s = C.cast(opens, C.c_void_p)
+        L.rate_hz, L.jitter, L.seed, L.keep_open = rate_hz, jitter, seed, int(keep_open)
+        tickets = (C.c_int * S)()
+        st = kf_server_stats()
+        check(lib().kf_server_run_load(self.h, C.byref(L), tickets, C.byref(st)), "kf_server_run_load")
+        for k in range(S):
+            self.configs[tickets[k]] = configs[k]
+        return list(tickets), {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().kf_server_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

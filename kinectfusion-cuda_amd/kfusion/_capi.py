@@ -109,6 +109,47 @@ class kf_batch_config(C.Structure):
     ]
 
 
+class kf_server_config(C.Structure):
+    _fields_ = [
+        ("batch", kf_batch_config),
+        ("hbm_budget_bytes", C.c_size_t),
+        ("queue_depth", C.c_int),
+    ]
+
+
+class kf_server_stats(C.Structure):
+    _fields_ = [
+        ("frames", C.c_int64),
+        ("ticks", C.c_int64),
+        ("admitted", C.c_int64),
+        ("queued_admissions", C.c_int64),
+        ("failed_frames", C.c_int64),
+        ("wall_s", C.c_double),
+        ("frames_per_s", C.c_double),
+        ("mean_frames_per_tick", C.c_double),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("latency_max_ms", C.c_double),
+        ("latency_mean_ms", C.c_double),
+        ("peak_resident_bytes", C.c_size_t),
+    ]
+
+
+class kf_poisson_load(C.Structure):
+    _fields_ = [
+        ("sessions", C.c_int),
+        ("configs", C.c_void_p),
+        ("banks", C.c_void_p),
+        ("bank_len", C.c_void_p),
+        ("frames_per_session", C.c_void_p),
+        ("opens_s", C.c_void_p),
+        ("rate_hz", C.c_double),
+        ("jitter", C.c_double),
+        ("seed", C.c_uint64),
+        ("keep_open", C.c_int),
+    ]
+
+
 P = C.c_void_p
 I = C.c_int
 F = C.c_float
@@ -187,6 +228,20 @@ SIGNATURES = {
     "kf_batch_shape_get": (I, [P, C.POINTER(kf_batch_shape), C.POINTER(I)]),
     "kf_batch_tick_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64), FP]),
     "kf_batch_touched_total": (I, [P, C.POINTER(C.c_ulonglong)]),
+    "kf_server_create": (I, [C.POINTER(kf_server_config), C.POINTER(P)]),
+    "kf_server_destroy": (I, [P]),
+    "kf_server_now": (C.c_double, []),
+    "kf_server_open": (I, [P, C.POINTER(kf_session_config), C.POINTER(I)]),
+    "kf_server_close": (I, [P, I]),
+    "kf_server_submit": (I, [P, I, P, C.c_double]),
+    "kf_server_poll": (I, [P, C.POINTER(I)]),
+    "kf_server_drain": (I, [P]),
+    "kf_server_stats_get": (I, [P, C.POINTER(kf_server_stats)]),
+    "kf_server_latencies": (I, [P, C.POINTER(C.c_double), SZ, C.POINTER(SZ)]),
+    "kf_server_state": (I, [P, I, C.POINTER(I)]),
+    "kf_server_pose": (I, [P, I, P]),
+    "kf_server_volume": (I, [P, I, C.POINTER(P)]),
+    "kf_server_run_load": (I, [P, C.POINTER(kf_poisson_load), C.POINTER(I), C.POINTER(kf_server_stats)]),
 }
 
 _lib = None
