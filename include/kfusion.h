@@ -308,6 +308,7 @@ typedef struct kf_server_config {
 typedef struct kf_server_stats {
     int64_t frames, ticks, admitted, queued_admissions, failed_frames;
     double wall_s, frames_per_s, mean_frames_per_tick;
+    double offered_frames_per_s;  /* submitted frames / (last arrival - first arrival) */
     double latency_p50_ms, latency_p99_ms, latency_max_ms, latency_mean_ms;
     size_t peak_resident_bytes;
 } kf_server_stats;

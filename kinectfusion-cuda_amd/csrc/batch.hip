@@ -105,6 +105,7 @@ struct kf_batch {
     Slot slot[2];
     int next = 0;
     unsigned* fuse_count = nullptr;
+    unsigned* ray_next = nullptr;  // 16 raycast tile counters
     unsigned* fuse_list = nullptr;
     size_t list_cap = 0;
     unsigned long long* touched = nullptr;
@@ -354,6 +355,7 @@ int kf_batch_destroy(kf_batch* b)
     for (Session& q : b->sess)
         if (q.used) free_session(q);
     (void)hipFree(b->fuse_count);
+    (void)hipFree(b->ray_next);
     (void)hipFree(b->fuse_list);
     (void)hipFree(b->touched);
     if (b->own_stream && b->s) (void)hipStreamDestroy(b->s);
@@ -405,6 +407,7 @@ int kf_batch_create(const kf_batch_config* cfg, kf_stream_t stream, kf_batch** o
             if (!st && hipEventCreate(&ev) != hipSuccess) st = kf::fail(KF_ERR_HIP, "kf_batch_create events");
     }
     if (!st) st = alloc((void**)&b->fuse_count, sizeof(unsigned), nullptr);
+    if (!st) st = alloc((void**)&b->ray_next, 16 * sizeof(unsigned), nullptr);
     if (!st) st = alloc((void**)&b->touched, sizeof(unsigned long long) * kMaxFuseBlocks, nullptr);
     if (!st) {
         hipError_t e = hipMemset(b->touched, 0, sizeof(unsigned long long) * kMaxFuseBlocks);
@@ -524,6 +527,7 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
     auto* fuse = reinterpret_cast<kf::FuseJob*>(base + b->off_fuse);
     memset(hd, 0, sizeof(*hd));
     hd->fuse_count = b->fuse_count;
+    hd->ray_next = b->ray_next;
     hd->fuse_list = b->fuse_list;
     hd->touched = b->touched;
     const kf_icp_config& ic = b->cfg.icp;

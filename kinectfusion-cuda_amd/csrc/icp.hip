@@ -678,20 +678,11 @@ __global__ void icp_begin_kernel(PoseState* st, IcpCarry* c0, Pose16 pose, int f
     begin_state(st, c0);
 }
 
-// Finishes the last iteration and commits the estimate as the sensor pose
-// (Sensor::setPose, kinect_fusion.cpp:146-161, sensor.h:30-34); on failure the
-// previous pose is kept and the frame is not fused.
-template <bool COMP>
-__device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin, float stop_angle, float stop_dist,
-                            double* tot, double* count)
+// Commits the carried estimate as the sensor pose (Sensor::setPose,
+// kinect_fusion.cpp:146-161, sensor.h:30-34); on failure the previous pose is
+// kept and the frame is not fused.  One lane.
+__device__ void commit_state(PoseState* st, const IcpCarry& c)
 {
-    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
-    if (threadIdx.x >= 64) return;
-    double xd[6];
-    const int rc = solve_pending(cin, tot, xd);
-    if (threadIdx.x != 0) return;
-    IcpCarry c = *cin;
-    apply_solution(c, rc, xd, tot, *count, st, true, stop_angle, stop_dist);
     for (int q = 0; q < 9; ++q) st->R[q] = c.R[q];
     for (int q = 0; q < 3; ++q) st->t[q] = c.t[q];
     for (int l = 0; l < 4; ++l) st->level_done[l] = c.level_done[l];
@@ -720,6 +711,21 @@ __device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin
     I[15] = 1.0f;
 }
 
+// Finishes the last iteration and commits the estimate.
+template <bool COMP>
+__device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin, float stop_angle, float stop_dist,
+                            double* tot, double* count)
+{
+    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
+    if (threadIdx.x >= 64) return;
+    double xd[6];
+    const int rc = solve_pending(cin, tot, xd);
+    if (threadIdx.x != 0) return;
+    IcpCarry c = *cin;
+    apply_solution(c, rc, xd, tot, *count, st, true, stop_angle, stop_dist);
+    commit_state(st, c);
+}
+
 template <bool COMP>
 __global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, const IcpCarry* cin, const float* pin,
                                                               float stop_angle, float stop_dist)
@@ -729,35 +735,65 @@ __global__ __launch_bounds__(kThreads) void icp_commit_kernel(PoseState* st, con
     commit_body<COMP>(st, cin, pin, stop_angle, stop_dist, tot, &count);
 }
 
-// ---- the same loop for many sessions per launch (batch ticks).  The jobs of
-// a level are sorted by block0; workgroup ranks own contiguous block ranges
-// (XCD-contiguous), and a workgroup solves a session's previous iteration the
-// first time it meets one of that session's blocks.  Per-session results do
-// not depend on the mix: partial rows are indexed by the session's own block
-// number and summed in that order.
-__device__ __forceinline__ const float* job_partials(const IcpJob& J, int k)
-{
-    return J.partials + (size_t)(k & 1) * kPartialStride * (size_t)J.max_blocks;
-}
-__device__ __forceinline__ float* job_partials_w(const IcpJob& J, int k)
-{
-    return J.partials + (size_t)(k & 1) * kPartialStride * (size_t)J.max_blocks;
-}
-
+// ---- the same loop for many sessions per launch (batch ticks).  Each
+// iteration is two launches: a solve launch (one workgroup per session)
+// finishes the previous iteration from the session's partial rows and puts the
+// pose of this iteration into its carry; an association launch reduces every
+// block of every session (workgroup ranks own contiguous, XCD-contiguous block
+// ranges).  Splitting keeps the FP64 solve out of the registers of the wide
+// launch.  Partial rows are indexed by the session's own block number and
+// summed in that order, so a session's result does not depend on the mix and
+// equals the single-session loop's.
 __global__ void icp_batch_begin_kernel(const BatchHeader* hd, const IcpJob* jobs)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < hd->n_icp) begin_state(jobs[q].a.st, jobs[q].carry);
 }
 
+// final = 0: finish the pending iteration and open `level`; final = 1: finish
+// the last iteration and commit the estimate (as icp_commit_kernel).
 template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_batch_step_kernel(const BatchHeader* hd, const IcpJob* jobs, int level,
-                                                                  int k, float stop_angle, float stop_dist)
+__global__ __launch_bounds__(kThreads) void icp_batch_solve_kernel(const BatchHeader* hd, const IcpJob* jobs,
+                                                                   int level, int final_, float stop_angle,
+                                                                   float stop_dist)
+{
+    __shared__ double tot[kTerms];
+    __shared__ double count;
+    for (int q = blockIdx.x; q < hd->n_icp; q += gridDim.x) {
+        const IcpJob& J = jobs[q];
+        IcpCarry* c = J.carry;
+        PoseState* st = J.a.st;
+        const bool pend = c->pending && c->status == 0;
+        __syncthreads();  // every lane has read the carry before lane 0 rewrites it
+        if (pend) sum_partials<COMP>(J.partials, c->pending_blocks, c->pending_blocks, tot, &count);
+        if (threadIdx.x < 64) {
+            double xd[6];
+            const int rc = pend ? chol6_wave(tot, xd) : 0;
+            if (threadIdx.x == 0) {
+                IcpCarry cc = *c;
+                apply_solution(cc, rc, xd, tot, count, st, true, stop_angle, stop_dist);
+                if (!final_) {
+                    cc.pending = (cc.status == 0 && !level_flag(cc.level_done, level)) ? 1 : 0;
+                    cc.pending_level = level;
+                    cc.pending_blocks = J.a.stride;
+                    *c = cc;
+                } else {
+                    cc.pending = 0;
+                    *c = cc;
+                    commit_state(st, cc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kThreads) void icp_batch_assoc_kernel(const BatchHeader* hd, const IcpJob* jobs,
+                                                                   int level)
 {
     __shared__ float pose[24];
     __shared__ int run;
-    __shared__ double tot[kTerms];
-    __shared__ double count;
     const int njobs = hd->n_icp, total = hd->icp_blocks[level];
     const int G = (int)gridDim.x;
     const int rank = xcd_rank((int)blockIdx.x, G);
@@ -765,37 +801,50 @@ __global__ __launch_bounds__(kThreads) void icp_batch_step_kernel(const BatchHea
     int b = rank * chunk;
     const int end = min(total, b + chunk);
     if (b >= end) return;
+    // the table is never written while the kernel runs: scalar loads through
+    // the constant address space keep the job's fields in scalar registers
+    typedef __attribute__((address_space(4))) const IcpJob ConstJob;
+    ConstJob* cj = (ConstJob*)jobs;
     int j = job_of(jobs, njobs, b, [](const IcpJob& J) { return J.block0; });
     int cur = -1;
     for (; b < end; ++b) {
-        while (j + 1 < njobs && jobs[j + 1].block0 <= b) ++j;
-        const IcpJob& J = jobs[j];
-        const int lb = b - J.block0;
+        while (j + 1 < njobs && cj[j + 1].block0 <= b) ++j;
+        j = __builtin_amdgcn_readfirstlane(j);
+        IcpArgs a;
+        a.V = cj[j].a.V;
+        a.TV = cj[j].a.TV;
+        a.TN = cj[j].a.TN;
+        a.w = cj[j].a.w;
+        a.h = cj[j].a.h;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.K[q] = cj[j].a.K[q];
+        a.dist_thresh = cj[j].a.dist_thresh;
+        a.cos_thresh = cj[j].a.cos_thresh;
+        a.st = cj[j].a.st;
+        a.level = level;
+        a.partials = cj[j].partials;
+        a.stride = cj[j].a.stride;
+        const int lb = b - cj[j].block0;
         LanePixels lp;
-        load_lane_pixels(J.a, lp, lb);
-        __syncthreads();  // the previous block's use of pose / the reduction LDS is over
+        load_lane_pixels(a, lp, lb);
         if (j != cur) {
             cur = j;
-            begin_iteration<COMP>(J.a.st, J.carry + (k & 1), J.carry + ((k + 1) & 1), job_partials(J, k + 1), level,
-                                  J.a.stride, lb == 0, stop_angle, stop_dist, pose, &run, tot, &count);
+            __syncthreads();  // the previous session's pose is no longer read
+            if (threadIdx.x == 0) {
+                const IcpCarry* c = cj[j].carry;
+                const PoseState* st = a.st;
+                for (int q = 0; q < 9; ++q) pose[q] = c->R[q];
+                for (int q = 0; q < 3; ++q) pose[9 + q] = c->t[q];
+                for (int q = 0; q < 9; ++q) pose[12 + q] = st->Rp[q];
+                for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
+                run = c->pending;
+            }
+            __syncthreads();
+        } else {
+            __syncthreads();  // the previous block's reduction LDS is free again
         }
         if (!run) continue;
-        IcpArgs a = J.a;
-        a.partials = job_partials_w(J, k);
         associate_reduce<COMP>(a, lp, pose, lb);
-    }
-}
-
-template <bool COMP>
-__global__ __launch_bounds__(kThreads) void icp_batch_commit_kernel(const BatchHeader* hd, const IcpJob* jobs, int k,
-                                                                    float stop_angle, float stop_dist)
-{
-    __shared__ double tot[kTerms];
-    __shared__ double count;
-    for (int q = blockIdx.x; q < hd->n_icp; q += gridDim.x) {
-        const IcpJob& J = jobs[q];
-        commit_body<COMP>(J.a.st, J.carry + (k & 1), job_partials(J, k + 1), stop_angle, stop_dist, tot, &count);
-        __syncthreads();
     }
 }
 
@@ -875,23 +924,23 @@ int launch_icp_batch(const BatchHeader* hd, const IcpJob* jobs, const int* iters
     icp_batch_begin_kernel<<<(unsigned)((max_sessions + 63) / 64), 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("icp_batch_begin_kernel");
     const IcpJob* level_jobs[3] = {jobs, jobs + max_sessions, jobs + 2 * max_sessions};
-    int k = 0;
+    const unsigned sb = (unsigned)std::min(max_sessions, 1024);
     for (int l = levels - 1; l >= 0; --l)
-        for (int it = 0; it < iters[l]; ++it, ++k) {
-            if (compensated)
-                icp_batch_step_kernel<true><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l, k, stop_angle,
-                                                                             stop_dist);
-            else
-                icp_batch_step_kernel<false><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l, k, stop_angle,
-                                                                              stop_dist);
-            KF_LAUNCHED("icp_batch_step_kernel");
+        for (int it = 0; it < iters[l]; ++it) {
+            if (compensated) {
+                icp_batch_solve_kernel<true><<<sb, kThreads, 0, s>>>(hd, level_jobs[l], l, 0, stop_angle, stop_dist);
+                icp_batch_assoc_kernel<true><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l);
+            } else {
+                icp_batch_solve_kernel<false><<<sb, kThreads, 0, s>>>(hd, level_jobs[l], l, 0, stop_angle, stop_dist);
+                icp_batch_assoc_kernel<false><<<step_blocks, kThreads, 0, s>>>(hd, level_jobs[l], l);
+            }
+            KF_LAUNCHED("icp_batch_assoc_kernel");
         }
-    const unsigned cb = (unsigned)std::min(max_sessions, 1024);
     if (compensated)
-        icp_batch_commit_kernel<true><<<cb, kThreads, 0, s>>>(hd, jobs, k, stop_angle, stop_dist);
+        icp_batch_solve_kernel<true><<<sb, kThreads, 0, s>>>(hd, jobs, 0, 1, stop_angle, stop_dist);
     else
-        icp_batch_commit_kernel<false><<<cb, kThreads, 0, s>>>(hd, jobs, k, stop_angle, stop_dist);
-    KF_LAUNCHED("icp_batch_commit_kernel");
+        icp_batch_solve_kernel<false><<<sb, kThreads, 0, s>>>(hd, jobs, 0, 1, stop_angle, stop_dist);
+    KF_LAUNCHED("icp_batch_solve_kernel");
     return KF_OK;
 }
 

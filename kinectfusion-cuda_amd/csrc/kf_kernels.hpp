@@ -29,6 +29,7 @@ struct BatchHeader {
     unsigned* fuse_count;
     unsigned* fuse_list;
     unsigned long long* touched;
+    unsigned* ray_next;  // 16 tile counters (8 XCD bands x 2 voxel formats), cleared by the first kernel
 };
 
 // Preprocess of one session: raw codes -> metres (fuse input), bilateral L0,

@@ -122,12 +122,15 @@ __global__ __launch_bounds__(256) void vertex_kernel(const float* __restrict__ d
 // Pass 1, per level-0 16x16 tile: raw codes -> metres in LDS (the fuse input is
 // written back, with the tile's max valid depth), bilateral filter, level-0
 // vertex map.  Also clears the tick's
-// fuse-list counter, which the fuse passes of this tick use after it.
+// fuse-list counter and raycast tile counters, which later passes of this tick use.
 __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
 {
     __shared__ BilateralTile tile;
     __shared__ float wave_max[4];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *hd->fuse_count = 0u;
+    if (blockIdx.x == 0 && threadIdx.y == 0) {
+        if (threadIdx.x == 0) *hd->fuse_count = 0u;
+        hd->ray_next[threadIdx.x] = 0u;  // kTile == 16 counters
+    }
     const int tiles = hd->pre_tiles, njobs = hd->n_pre;
     const int t = threadIdx.y * kTile + threadIdx.x;
     int jb = 0;

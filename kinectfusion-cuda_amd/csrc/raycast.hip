@@ -177,10 +177,46 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
     return (1.0f - c) * g0 + c * g1;
 }
 
-// raycast.cu:44-145 for pixel (i, j) of one level map.
+// Small steps a lane marches on its own before its ray is handed to the
+// whole wave (about 1 ray in 40 grazes a surface and takes hundreds).
+constexpr int kSmallBudget = 32;
+
+// One trilinear evaluation of the march at t: 1 = skipped (the sample is known
+// to be >= 0.99), 0 = evaluated into f.
 template <int FMT>
-__device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i, int j)
+__device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir, float t, float width, float invw,
+                                            float nf, int& cs, float& cb, int& ck, float& cbk, CellCache& cc,
+                                            float& f)
 {
+    const v3 p = add3(o, mul3s(dir, t));
+    const int fs = a.brick_min ? fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
+                                           (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
+                               : 0;
+    if (fs == 1) return 1;
+    const Stencil st = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+    if (fs < 0 && skippable(a, st)) return 1;
+    f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
+    return 0;
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// raycast.cu:44-145 for pixel (i, j) of one level map.  Called by all 64
+// lanes of a wave together (in_image false for lanes past the map's edge).
+// Phase 1: each lane marches its own ray; a ray still marching after
+// kSmallBudget small steps is deferred.  Phase 2: the wave marches each
+// deferred ray 64 samples at a time, lane k taking sample k of the chunk (its
+// position by the same repeated additions t = t + step), and a ballot finds
+// the first negative sample.  Every sample position, value and decision is the
+// sequential march's, so the output is the same.
+template <int FMT>
+__device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i,
+                                              int j, bool in_image)
+{
+    const int lane = threadIdx.x & 63;
     float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     float M[9];
     m3m3(Rm, L.Kinv, M);
@@ -202,62 +238,118 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         tmin = -1.0f;
         tmax = -1.0f;
     }
-    // raycast.cu:61-73: leave the pixel untouched
-    if (tmax < 0.0f) return;
     const float nearv = (kMinDistance < tmin) ? tmin : kMinDistance;
     const float farv = (tmax < kMaxDistance) ? tmax : kMaxDistance;
-    if (nearv >= farv) return;
+    // raycast.cu:61-73: a ray behind the camera or with near >= far leaves the pixel untouched
+    const bool active = in_image && !(tmax < 0.0f) && nearv < farv;
 
     // raycast.cu:75-117.  A skipped sample only sets prev; its value is
     // computed if and when the zero-crossing formula needs it.
     const float large = mue * 1.99f, small = mue * 0.1f;
-    float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f;
-    bool prev_skipped = false;
     const float invw = 1.0f / width, nf = (float)a.n;
+    float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f, t = nearv;
+    bool prev_skipped = false, deferred = false;
     int cs = -1, ck = -1;
     float cb = 0.0f, cbk = 0.0f;
     CellCache cc;
     int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
-    for (float t = nearv; t < farv; t = t + step) {
-        if (++guard > kMaxMarchSteps) break;
-        v3 p = add3(o, mul3s(dir, t));
-        const int fs = a.brick_min ? fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
-                                               (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
-                                   : 0;
-        if (fs == 1) {
-            prev_skipped = true;
-            prev_t = t;
-            continue;
-        }
-        const Stencil st = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
-        if (fs < 0 && skippable(a, st)) {
-            prev_skipped = true;
-            prev_t = t;
-            continue;
-        }
-        float f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
-        if (f < 0.0f) {
-            if (t != nearv) {
-                if (step == large) {
-                    t = t - mue;
-                    step = small;
-                    continue;
-                }
-                if (prev_skipped) {
-                    v3 pp = add3(o, mul3s(dir, prev_t));
-                    prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
-                                            pp.z / width + 0.5f);
-                }
-                tstar = t - step * prev / (f - prev);
+    if (active) {
+        int smalls = 0;
+        for (; t < farv; t = t + step) {
+            if (++guard > kMaxMarchSteps) break;
+            if (step == small && ++smalls > kSmallBudget) {
+                deferred = true;  // this sample (guard already counted) is the wave's first
+                break;
             }
-            break;
-        } else if (f < 0.99f) {
-            step = small;
+            float f = 0.0f;
+            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, cs, cb, ck, cbk, cc, f)) {
+                prev_skipped = true;
+                prev_t = t;
+                continue;
+            }
+            if (f < 0.0f) {
+                if (t != nearv) {
+                    if (step == large) {
+                        t = t - mue;
+                        step = small;
+                        continue;
+                    }
+                    if (prev_skipped) {
+                        v3 pp = add3(o, mul3s(dir, prev_t));
+                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
+                                                pp.z / width + 0.5f);
+                    }
+                    tstar = t - step * prev / (f - prev);
+                }
+                break;
+            } else if (f < 0.99f) {
+                step = small;
+            }
+            prev = f;
+            prev_skipped = false;
         }
-        prev = f;
-        prev_skipped = false;
     }
 
+    // phase 2: the deferred rays of the wave, one at a time, 64 samples per chunk
+    uint64_t pending = __ballot(deferred);
+    while (pending) {
+        const int r = __ffsll((unsigned long long)pending) - 1;
+        pending &= pending - 1;
+        const v3 dr = mk3(lane_bcast(dir.x, r), lane_bcast(dir.y, r), lane_bcast(dir.z, r));
+        const float fr = lane_bcast(farv, r);
+        float tb = lane_bcast(t, r);  // position of sample 0 of the chunk
+        float pv = lane_bcast(prev, r), pt = lane_bcast(prev_t, r);
+        int psk = __builtin_amdgcn_readlane(prev_skipped ? 1 : 0, r);
+        int gb = __builtin_amdgcn_readlane(guard, r);  // guard count of sample 0
+        float ts = 0.0f;
+        for (;;) {
+            float tk = tb;
+            for (int q = 0; q < lane; ++q) tk = tk + small;
+            // sample k exists if every sample up to it passes the loop test and the guard
+            const bool end = !(tk < fr) || gb + lane > kMaxMarchSteps;
+            float f = 0.0f;
+            int sk = 1;
+            if (!end) sk = march_sample<FMT>(a, o, dr, tk, width, invw, nf, cs, cb, ck, cbk, cc, f);
+            const bool neg = !end && !sk && f < 0.0f;
+            const uint64_t ev = __ballot(end || neg);
+            if (ev) {
+                const int k = __ffsll((unsigned long long)ev) - 1;
+                const bool kneg = __builtin_amdgcn_readlane(neg ? 1 : 0, k) != 0;
+                if (kneg) {
+                    // prev = the value of sample k - 1 (exactly, also if it was skipped)
+                    float pk;
+                    if (k > 0) {
+                        float fe = f;
+                        if (lane == k - 1 && sk) {
+                            v3 pp = add3(o, mul3s(dr, tk));
+                            fe = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
+                                                  pp.z / width + 0.5f);
+                        }
+                        pk = lane_bcast(fe, k - 1);
+                    } else if (psk) {
+                        v3 pp = add3(o, mul3s(dr, pt));
+                        pk = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
+                                              pp.z / width + 0.5f);
+                    } else {
+                        pk = pv;
+                    }
+                    const float tsk = tk - small * pk / (f - pk);
+                    ts = lane_bcast(tsk, k);
+                }
+                break;
+            }
+            // no event: the chunk's last sample becomes prev, the next chunk follows it
+            const float t63 = lane_bcast(tk, 63);
+            psk = __builtin_amdgcn_readlane(sk, 63);
+            pv = lane_bcast(f, 63);
+            pt = t63;
+            tb = t63 + small;
+            gb += 64;
+        }
+        if (lane == r) tstar = ts;
+    }
+
+    if (!active) return;
     const int64_t q = (int64_t)j * L.w + i;
     if (tstar != 0.0f) {
         v3 vtx = add3(o, mul3s(dir, tstar));
@@ -301,7 +393,7 @@ __device__ __forceinline__ int xcd_tile(int tiles)
 }
 
 template <int FMT>
-__global__ __launch_bounds__(64, 8) void raycast_kernel(RaycastArgs a)
+__global__ __launch_bounds__(64, 7) void raycast_kernel(RaycastArgs a)
 {
     const int tile = xcd_tile(a.tiles);
     if (tile >= a.tiles) return;
@@ -311,8 +403,7 @@ __global__ __launch_bounds__(64, 8) void raycast_kernel(RaycastArgs a)
     const int lt = tile - L.tile0;
     const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
     const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-    if (i >= L.w || j >= L.h) return;
-    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j);
+    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h);
 }
 
 // Field-wise copy of a job read through the constant address space.
@@ -340,38 +431,55 @@ __device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, Raycas
 
 // Many sessions and levels per persistent launch, one launch per voxel format:
 // the jobs of format FMT are jobs[j0, j1), covering tiles [t0, t1) of the tick
-// (jobs[k] covers [jobs[k].L.tile0, jobs[k+1].L.tile0)); workgroup ranks take
-// contiguous tile bands (XCD-contiguous).
+// (jobs[k] covers [jobs[k].L.tile0, jobs[k+1].L.tile0)).  March lengths vary
+// by an order of magnitude between tiles (rays grazing a surface take hundreds
+// of small steps), so tiles are claimed dynamically: the range is cut into 8
+// bands, workgroups on XCD x claim pairs of tiles of band x from its counter
+// (locality in that XCD's L2) and then help the other bands until all are done.
+constexpr int kClaim = 2;  // tiles per claim
+
 template <int FMT>
-__global__ __launch_bounds__(64, 8) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+__global__ __launch_bounds__(64, 7) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
 {
     const int nf = hd->n_ray_f32, tf = hd->ray_tiles_f32;
     const int j0 = FMT == KF_VOXEL_F32 ? 0 : nf, j1 = FMT == KF_VOXEL_F32 ? nf : hd->n_ray;
     const int t0 = FMT == KF_VOXEL_F32 ? 0 : tf, t1 = FMT == KF_VOXEL_F32 ? tf : hd->ray_tiles;
-    const int G = (int)gridDim.x;
-    const int rank = xcd_rank((int)blockIdx.x, G);
-    const int chunk = (t1 - t0 + G - 1) / G;
-    const int end = min(t1, t0 + (rank + 1) * chunk);
-    int t = t0 + rank * chunk;
-    if (t >= end) return;
+    if (t1 <= t0) return;
+    unsigned* next = hd->ray_next + (FMT == KF_VOXEL_F32 ? 0 : 8);
+    const int per = (t1 - t0 + 7) / 8;
+    const int xcd = (int)(blockIdx.x & 7);
     // The table is read through the constant address space: it is never written
     // while the kernel runs, so its fields load with scalar loads into scalar
     // registers even after the kernel's stores (the vertex and normal maps).
     typedef __attribute__((address_space(4))) const RaycastJob ConstJob;
     ConstJob* cj = (ConstJob*)jobs;
-    int jb = j0 + job_of(jobs + j0, j1 - j0, t, [](const RaycastJob& x) { return x.L.tile0; });
-    for (; t < end; ++t) {
-        while (jb + 1 < j1 && cj[jb + 1].L.tile0 <= t) ++jb;
-        jb = __builtin_amdgcn_readfirstlane(jb);
-        RaycastVolume vol;
-        RaycastLevel L;
-        copy_job(cj[jb], vol, L);
-        const float* pose = cj[jb].pose;
-        const int lt = t - L.tile0;
-        const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
-        const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-        if (i >= L.w || j >= L.h) continue;
-        raycast_pixel<FMT>(vol, pose, L, i, j);
+    for (int bi = 0; bi < 8; ++bi) {
+        const int band = (xcd + bi) & 7;
+        const int b0 = t0 + band * per, b1 = min(t1, b0 + per);
+        for (;;) {
+            int c = 0;
+            if (threadIdx.x == 0) c = (int)atomicAdd(&next[band], 1u);
+            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+            const int tc = b0 + c * kClaim;
+            if (tc >= b1) break;
+            for (int t = tc; t < min(b1, tc + kClaim); ++t) {
+                int lo = j0, hi = j1 - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (cj[mid].L.tile0 <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int jb = __builtin_amdgcn_readfirstlane(lo);
+                RaycastVolume vol;
+                RaycastLevel L;
+                copy_job(cj[jb], vol, L);
+                const float* pose = cj[jb].pose;
+                const int lt = t - L.tile0;
+                const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
+                const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
+                raycast_pixel<FMT>(vol, pose, L, i, j, i < L.w && j < L.h);
+            }
+        }
     }
 }
 

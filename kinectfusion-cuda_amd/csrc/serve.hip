@@ -77,7 +77,8 @@ struct kf_server {
     int next = 0;
     std::vector<double> lat;
     int64_t ticks = 0, frames_in_ticks = 0, admitted = 0, queued_admissions = 0, failed = 0;
-    double t_first = -1.0, t_last = 0.0;
+    double t_first = -1.0, t_last = 0.0, t_last_arrival = 0.0;
+    int64_t submitted = 0;
 };
 
 namespace {
@@ -284,6 +285,8 @@ int kf_server_submit(kf_server* s, int id, const uint16_t* frame, double arrival
     q->arrival[slot] = arrival_s;
     q->count += 1;
     if (s->t_first < 0.0 || arrival_s < s->t_first) s->t_first = arrival_s;
+    if (arrival_s > s->t_last_arrival) s->t_last_arrival = arrival_s;
+    s->submitted += 1;
     return KF_OK;
 }
 
@@ -338,6 +341,8 @@ int kf_server_stats_get(kf_server* s, kf_server_stats* o)
     o->wall_s = (s->t_first >= 0.0 && s->t_last > s->t_first) ? s->t_last - s->t_first : 0.0;
     o->frames_per_s = o->wall_s > 0 ? (double)o->frames / o->wall_s : 0.0;
     o->mean_frames_per_tick = s->ticks ? (double)s->frames_in_ticks / (double)s->ticks : 0.0;
+    const double span = s->t_last_arrival - s->t_first;
+    o->offered_frames_per_s = (s->t_first >= 0.0 && span > 0.0) ? (double)s->submitted / span : 0.0;
     o->latency_p50_ms = percentile(s->lat, 50.0);
     o->latency_p99_ms = percentile(s->lat, 99.0);
     double mx = 0.0, sum = 0.0;

@@ -127,6 +127,7 @@ class kf_server_stats(C.Structure):
         ("wall_s", C.c_double),
         ("frames_per_s", C.c_double),
         ("mean_frames_per_tick", C.c_double),
+        ("offered_frames_per_s", C.c_double),
         ("latency_p50_ms", C.c_double),
         ("latency_p99_ms", C.c_double),
         ("latency_max_ms", C.c_double),
