@@ -121,6 +121,19 @@ def batched(args, kfusion, stream):
                                           "seeds_over_5cm": [100 + s for s in range(S) if errs[s] > 0.05]}}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the fuse sweep on this workload, from the counter runs summarised
+    in profiles/fuse_list_kernel_traffic_plain512.md (FETCH_SIZE and WRITE_SIZE scaled by a
+    known-size copy); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "fuse_list_kernel_traffic_plain512.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch fuse_list_kernel<F32> plain-640x480-512:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(seq, raws, n, L, mue, budget_s):
     """The oracle (CPU restatement of the same path, OpenMP over pixels/voxels) on the
     first frames of the same sequence, bounded to about budget_s seconds."""
@@ -280,7 +293,7 @@ def main():
                    "final_pose_err_vs_truth_m": round(float(np.abs(pose[12:15] - gt[12:15]).max()), 5)},
         "roofline": {"bound": "hbm", "kernel": "fuse_list_kernel<F32> (voxel sweep; the brick cull and depth tile-max are two short launches before it)", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(fuse_ms.value, 5),
+                     "traffic": measured_traffic(), "avg_launch_ms": round(fuse_ms.value, 5),
                      "touched_voxels_per_launch": round(touched_per), "algorithmic_bytes_per_launch": round(alg_bytes)},
     }
     lib.kf_pipeline_destroy(h)

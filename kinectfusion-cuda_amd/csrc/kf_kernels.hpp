@@ -40,6 +40,7 @@ struct PreJob {
     float* d0;         // bilateral-filtered level 0
     float* dmax;       // tile-max maps of depth_m (FuseArgs::dmax layout)
     float4* v[3];      // camera-space vertex maps
+    float* d12[2];     // filtered depth of levels 1 and 2 (optional)
     int w, h;
     float scale;
     float Kinv[3][9];
@@ -47,6 +48,8 @@ struct PreJob {
     int ptile0, ptiles_x;  // level-2 16x16 tiles of this job
 };
 int launch_preprocess_batch(const BatchHeader* hd, const PreJob* jobs, unsigned blocks, hipStream_t s);
+// The same two passes for one session (w, h divisible by 4; no mirror).
+int launch_measure(const PreJob& J, hipStream_t s);
 
 // ---- TSDF (tsdf.hip)
 constexpr unsigned kFuseBlocks = 2048;  // fuse workgroups: 256 CUs x 8 resident workgroups of 4 waves
@@ -88,8 +91,12 @@ int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_
 // inv_pose: by-value world->camera matrix; dyn_state (may be null) overrides it
 // with PoseState::inv_pose and skips the launch's work when the frame failed.
 // list_events (optional): two events recorded around the brick-list kernel alone.
+// dmax_ready: the volume's tile-max maps (volume_depth_max) already hold this
+// depth map's (the measurement pass wrote them).
 int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
-                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events);
+                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events, bool dmax_ready);
+// The volume's tile-max map buffer sized for w x h (grown if needed); null on failure.
+float* volume_depth_max(kf_volume* v, int w, int h, hipStream_t s);
 int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s);
 // kf_volume_create without the host wait: initialisation is enqueued on `s`.
 int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, kf_volume** out);

@@ -118,11 +118,86 @@ __global__ __launch_bounds__(256) void vertex_kernel(const float* __restrict__ d
     out[(int64_t)j * w + i] = vertex_at(depth[(int64_t)j * w + i], i, j, ki.m);
 }
 
-// ---- many sessions per launch (batch ticks), persistent over the tick's tiles.
-// Pass 1, per level-0 16x16 tile: raw codes -> metres in LDS (the fuse input is
-// written back, with the tile's max valid depth), bilateral filter, level-0
-// vertex map.  Also clears the tick's
-// fuse-list counter and raycast tile counters, which later passes of this tick use.
+// ---- the measurement step in two launches (one session or many).
+// Pass 1, per level-0 16x16 tile lt of job J: raw codes -> metres in LDS (the
+// fuse input is written back, with the tile's max valid depth), bilateral
+// filter, level-0 vertex map.
+__device__ __forceinline__ void measure_tile(const PreJob& J, int lt, BilateralTile& tile, float* wave_max)
+{
+    const int t = threadIdx.y * kTile + threadIdx.x;
+    const int i0 = (lt % J.tiles_x) * kTile - 3, j0 = (lt / J.tiles_x) * kTile - 3;
+    __syncthreads();  // the previous tile's reads of the window are over
+    for (int q = t; q < (kTile + 6) * (kTile + 6); q += kTile * kTile) {
+        int ly = q / (kTile + 6), lx = q % (kTile + 6);
+        int gi = clampi(i0 + lx, 0, J.w - 1), gj = clampi(j0 + ly, 0, J.h - 1);
+        tile[ly][lx] = half_rne(J.raw[(int64_t)gj * J.w + gi]) * J.scale;
+    }
+    __syncthreads();
+    const int i = i0 + 3 + threadIdx.x, j = j0 + 3 + threadIdx.y;
+    float dv = 0.0f;  // this pixel's contribution to the tile's max valid depth
+    if (i < J.w && j < J.h) {
+        const int64_t q = (int64_t)j * J.w + i;
+        const float dm = tile[threadIdx.y + 3][threadIdx.x + 3];
+        J.depth_m[q] = dm;
+        dv = is_depth_valid(dm) ? dm : 0.0f;
+        const float d = bilateral_at(tile, threadIdx.x, threadIdx.y);
+        J.d0[q] = d;
+        J.v[0][q] = vertex_at(d, i, j, J.Kinv[0]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dv = fmaxf(dv, __shfl_xor(dv, off, 64));
+    if ((t & 63) == 0) wave_max[t >> 6] = dv;
+    __syncthreads();
+    if (t == 0) J.dmax[lt] = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));
+}
+
+// Pass 2, one thread per level-2 pixel of level-2 tile lt: its 4x4 level-0
+// block gives 4 level-1 depths (downSample) and those the level-2 depth; vertex
+// maps of both levels (and the depths, when J.d12 is set).  A workgroup covers
+// a 64x64 level-0 region and also writes its max valid depth.
+__device__ __forceinline__ void pyramid_tile(const PreJob& J, int lt)
+{
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int tx16 = J.tiles_x, ty16 = (J.h + 15) / 16;
+        const int rx = lt % J.ptiles_x, ry = lt / J.ptiles_x;
+        float m = 0.0f;
+        for (int qy = ry * 4; qy < min(ry * 4 + 4, ty16); ++qy)
+            for (int qx = rx * 4; qx < min(rx * 4 + 4, tx16); ++qx) m = fmaxf(m, J.dmax[qy * tx16 + qx]);
+        J.dmax[tx16 * ty16 + lt] = m;
+    }
+    const int i2 = (lt % J.ptiles_x) * kTile + threadIdx.x, j2 = (lt / J.ptiles_x) * kTile + threadIdx.y;
+    const int w1 = J.w >> 1, w2 = J.w >> 2;
+    if (i2 >= w2 || j2 >= (J.h >> 2)) return;
+    float l1[2][2];
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int i1 = 2 * i2 + dx, j1 = 2 * j2 + dy;
+            const float* r0 = J.d0 + (int64_t)(2 * j1) * J.w + 2 * i1;
+            const float* r1 = r0 + J.w;
+            const float d = down4(r0[0], r0[1], r1[0], r1[1]);
+            l1[dy][dx] = d;
+            J.v[1][(int64_t)j1 * w1 + i1] = vertex_at(d, i1, j1, J.Kinv[1]);
+            if (J.d12[0]) J.d12[0][(int64_t)j1 * w1 + i1] = d;
+        }
+    const float d2 = down4(l1[0][0], l1[0][1], l1[1][0], l1[1][1]);
+    J.v[2][(int64_t)j2 * w2 + i2] = vertex_at(d2, i2, j2, J.Kinv[2]);
+    if (J.d12[1]) J.d12[1][(int64_t)j2 * w2 + i2] = d2;
+}
+
+// One session: one workgroup per tile, the job by value.
+__global__ __launch_bounds__(256) void measure_kernel(PreJob J)
+{
+    __shared__ BilateralTile tile;
+    __shared__ float wave_max[4];
+    measure_tile(J, blockIdx.x, tile, wave_max);
+}
+__global__ __launch_bounds__(256) void pyramid_kernel(PreJob J) { pyramid_tile(J, blockIdx.x); }
+
+// Many sessions per launch (batch ticks), persistent over the tick's tiles.
+// The first kernel also clears the tick's fuse-list counter and raycast tile
+// counters, which later passes of this tick use.
 __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
 {
     __shared__ BilateralTile tile;
@@ -132,74 +207,18 @@ __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* h
         hd->ray_next[threadIdx.x] = 0u;  // kTile == 16 counters
     }
     const int tiles = hd->pre_tiles, njobs = hd->n_pre;
-    const int t = threadIdx.y * kTile + threadIdx.x;
-    int jb = 0;
     for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
-        jb = job_of(jobs, njobs, tl, [](const PreJob& x) { return x.tile0; });
-        const PreJob& J = jobs[jb];
-        const int lt = tl - J.tile0;
-        const int i0 = (lt % J.tiles_x) * kTile - 3, j0 = (lt / J.tiles_x) * kTile - 3;
-        __syncthreads();  // the previous tile's reads of the window are over
-        for (int q = t; q < (kTile + 6) * (kTile + 6); q += kTile * kTile) {
-            int ly = q / (kTile + 6), lx = q % (kTile + 6);
-            int gi = clampi(i0 + lx, 0, J.w - 1), gj = clampi(j0 + ly, 0, J.h - 1);
-            tile[ly][lx] = half_rne(J.raw[(int64_t)gj * J.w + gi]) * J.scale;
-        }
-        __syncthreads();
-        const int i = i0 + 3 + threadIdx.x, j = j0 + 3 + threadIdx.y;
-        float dv = 0.0f;  // this pixel's contribution to the tile's max valid depth
-        if (i < J.w && j < J.h) {
-            const int64_t q = (int64_t)j * J.w + i;
-            const float dm = tile[threadIdx.y + 3][threadIdx.x + 3];
-            J.depth_m[q] = dm;
-            dv = is_depth_valid(dm) ? dm : 0.0f;
-            const float d = bilateral_at(tile, threadIdx.x, threadIdx.y);
-            J.d0[q] = d;
-            J.v[0][q] = vertex_at(d, i, j, J.Kinv[0]);
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) dv = fmaxf(dv, __shfl_xor(dv, off, 64));
-        if ((t & 63) == 0) wave_max[t >> 6] = dv;
-        __syncthreads();
-        if (t == 0) J.dmax[lt] = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));
+        const PreJob& J = jobs[job_of(jobs, njobs, tl, [](const PreJob& x) { return x.tile0; })];
+        measure_tile(J, tl - J.tile0, tile, wave_max);
     }
 }
 
-// Pass 2, one thread per level-2 pixel: its 4x4 level-0 block gives 4 level-1
-// depths (downSample) and those the level-2 depth; vertex maps of both levels.
-// A workgroup covers a 64x64 level-0 region and also writes its max valid depth.
 __global__ __launch_bounds__(256) void pyramid_batch_kernel(const BatchHeader* hd, const PreJob* jobs)
 {
     const int tiles = hd->pyr_tiles, njobs = hd->n_pre;
     for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
         const PreJob& J = jobs[job_of(jobs, njobs, tl, [](const PreJob& x) { return x.ptile0; })];
-        const int lt = tl - J.ptile0;
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            // the 64x64 region of this tile: max of its 16x16 tiles' maxima
-            const int tx16 = J.tiles_x, ty16 = (J.h + 15) / 16;
-            const int rx = lt % J.ptiles_x, ry = lt / J.ptiles_x;
-            float m = 0.0f;
-            for (int qy = ry * 4; qy < min(ry * 4 + 4, ty16); ++qy)
-                for (int qx = rx * 4; qx < min(rx * 4 + 4, tx16); ++qx) m = fmaxf(m, J.dmax[qy * tx16 + qx]);
-            J.dmax[tx16 * ty16 + lt] = m;
-        }
-        const int i2 = (lt % J.ptiles_x) * kTile + threadIdx.x, j2 = (lt / J.ptiles_x) * kTile + threadIdx.y;
-        const int w1 = J.w >> 1, w2 = J.w >> 2;
-        if (i2 >= w2 || j2 >= (J.h >> 2)) continue;
-        float l1[2][2];
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const int i1 = 2 * i2 + dx, j1 = 2 * j2 + dy;
-                const float* r0 = J.d0 + (int64_t)(2 * j1) * J.w + 2 * i1;
-                const float* r1 = r0 + J.w;
-                const float d = down4(r0[0], r0[1], r1[0], r1[1]);
-                l1[dy][dx] = d;
-                J.v[1][(int64_t)j1 * w1 + i1] = vertex_at(d, i1, j1, J.Kinv[1]);
-            }
-        const float d2 = down4(l1[0][0], l1[0][1], l1[1][0], l1[1][1]);
-        J.v[2][(int64_t)j2 * w2 + i2] = vertex_at(d2, i2, j2, J.Kinv[2]);
+        pyramid_tile(J, tl - J.ptile0);
     }
 }
 
@@ -245,6 +264,16 @@ int launch_normal_map(const float4* v, float4* out, int w, int h, hipStream_t s)
 {
     normal_kernel<<<grid2d(w, h), dim3(kTile, kTile), 0, s>>>(v, out, w, h);
     KF_LAUNCHED("normal_kernel");
+    return KF_OK;
+}
+
+int launch_measure(const PreJob& J, hipStream_t s)
+{
+    const int tiles = J.tiles_x * ((J.h + 15) / 16), ptiles = J.ptiles_x * ((J.h / 4 + 15) / 16);
+    measure_kernel<<<(unsigned)tiles, dim3(kTile, kTile), 0, s>>>(J);
+    KF_LAUNCHED("measure_kernel");
+    pyramid_kernel<<<(unsigned)ptiles, dim3(kTile, kTile), 0, s>>>(J);
+    KF_LAUNCHED("pyramid_kernel");
     return KF_OK;
 }
 

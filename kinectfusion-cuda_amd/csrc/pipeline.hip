@@ -25,6 +25,7 @@ struct kf_pipeline {
     float K[3][9], Ki[3][9];
     int levels = 3;
     int64_t frame = 0;
+    bool normals_fresh = false;  // npyr holds the normals of the current vertex pyramid
     // timing: 7 events per frame (start, after preprocess, after raycast, after ICP, after fuse,
     // and the pair around the fuse's brick-list kernel)
     bool timing = false;
@@ -159,19 +160,47 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
     }
     int st;
     kf::PoseState* state = p->icp->state;
+    // the fused two-pass measurement (metres, bilateral, pyramid, vertex maps and
+    // the fuse's tile-max maps) when the frame allows it; else the stage kernels
+    const bool fused = !p->cfg.unmirror && p->levels == 3 && w % 4 == 0 && h % 4 == 0;
+    if (fused) {
+        kf::PreJob J;
+        memset(&J, 0, sizeof(J));
+        J.raw = raw;
+        J.depth_m = p->depth_m;
+        J.d0 = p->dpyr[0];
+        J.dmax = kf::volume_depth_max(p->vol, w, h, s);
+        if (!J.dmax) return kf::fail(KF_ERR_OOM, "kf_pipeline_frame: depth tile-max maps");
+        for (int l = 0; l < 3; ++l) J.v[l] = p->vpyr[l];
+        J.d12[0] = p->dpyr[1];
+        J.d12[1] = p->dpyr[2];
+        J.w = w;
+        J.h = h;
+        J.scale = p->cfg.depth_scale;
+        memcpy(J.Kinv, p->Ki, sizeof(J.Kinv));
+        J.tiles_x = (w + 15) / 16;
+        J.ptiles_x = (w / 4 + 15) / 16;
+        if ((st = kf::launch_measure(J, s))) return st;
+        p->normals_fresh = false;
+    }
     if (p->frame == 0) {
         // initializeTracking (kinect_fusion.cpp:75-94): convert and fuse at the initial pose
-        if ((st = kf::launch_convert(raw, p->depth_m, w, h, p->cfg.depth_scale, p->cfg.unmirror, s))) return st;
+        if (!fused && (st = kf::launch_convert(raw, p->depth_m, w, h, p->cfg.depth_scale, p->cfg.unmirror, s)))
+            return st;
         if (p->timing) {
             KF_HIP(hipEventRecord(ev[1], s));
             KF_HIP(hipEventRecord(ev[2], s));
             KF_HIP(hipEventRecord(ev[3], s));
         }
     } else {
-        // depthFrameToVertexPyramid (kinect_fusion.cpp:116-135) + normal pyramid for ICP
-        if ((st = kf::launch_preprocess(raw, w, h, p->cfg.depth_scale, p->cfg.unmirror, p->levels, p->Ki, p->depth_m,
-                                        p->dpyr, p->vpyr, p->npyr, s)))
-            return st;
+        // depthFrameToVertexPyramid (kinect_fusion.cpp:116-135); ICP computes the live
+        // normals itself, so the normal pyramid is only built when a caller asks for it
+        if (!fused) {
+            if ((st = kf::launch_preprocess(raw, w, h, p->cfg.depth_scale, p->cfg.unmirror, p->levels, p->Ki,
+                                            p->depth_m, p->dpyr, p->vpyr, nullptr, s)))
+                return st;
+            p->normals_fresh = false;
+        }
         if (p->timing) KF_HIP(hipEventRecord(ev[1], s));
         // raycastTsdf (kinect_fusion.cpp:137-144) at the previous pose
         if ((st = kf::launch_raycast_levels(p->vol, nullptr, state, p->levels, p->Ki, p->pv, p->pn, w, h, s)))
@@ -188,7 +217,8 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
         if (p->timing) KF_HIP(hipEventRecord(ev[3], s));
     }
     // fuseCurrentDepthToTSDF (kinect_fusion.cpp:179-183): unfiltered metres at the new pose
-    if ((st = kf::launch_fuse(p->depth_m, w, h, p->vol, p->K[0], nullptr, state, s, p->timing ? ev + 5 : nullptr)))
+    if ((st = kf::launch_fuse(p->depth_m, w, h, p->vol, p->K[0], nullptr, state, s, p->timing ? ev + 5 : nullptr,
+                              fused)))
         return st;
     if (p->timing) {
         KF_HIP(hipEventRecord(ev[4], s));
@@ -237,7 +267,18 @@ int kf_pipeline_map(kf_pipeline* p, int which, int level, void** dev_ptr, int* w
         case 0: ptr = p->depth_m; level = 0; break;
         case 1: ptr = p->dpyr[level]; break;
         case 2: ptr = p->vpyr[level]; break;
-        case 3: ptr = p->npyr[level]; break;
+        case 3:
+            // computed on request (createNormalMap, measurement.cu:223-242) from the current vertex maps
+            if (!p->normals_fresh) {
+                for (int l = 0; l < p->levels; ++l) {
+                    int st = kf::launch_normal_map(p->vpyr[l], p->npyr[l], p->cfg.frame_width >> l,
+                                                   p->cfg.frame_height >> l, p->s);
+                    if (st) return st;
+                }
+                p->normals_fresh = true;
+            }
+            ptr = p->npyr[level];
+            break;
         case 4: ptr = p->pv[level]; break;
         case 5: ptr = p->pn[level]; break;
         default: return kf::fail(KF_ERR_ARG, "kf_pipeline_map: unknown map");

@@ -177,9 +177,12 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
     return (1.0f - c) * g0 + c * g1;
 }
 
-// Small steps a lane marches on its own before its ray is handed to the
-// whole wave (about 1 ray in 40 grazes a surface and takes hundreds).
+// A ray that has taken kSmallBudget small steps is handed to the whole wave
+// once no more than kCoopLanes lanes of the wave are still marching (about 1
+// ray in 40 grazes a surface and takes hundreds of small steps; while many
+// lanes march, the wave is busy anyway).
 constexpr int kSmallBudget = 32;
+constexpr int kCoopLanes = 8;
 
 // One trilinear evaluation of the march at t: 1 = skipped (the sample is known
 // to be >= 0.99), 0 = evaluated into f.
@@ -206,8 +209,8 @@ __device__ __forceinline__ float lane_bcast(float v, int lane)
 
 // raycast.cu:44-145 for pixel (i, j) of one level map.  Called by all 64
 // lanes of a wave together (in_image false for lanes past the map's edge).
-// Phase 1: each lane marches its own ray; a ray still marching after
-// kSmallBudget small steps is deferred.  Phase 2: the wave marches each
+// Phase 1: each lane marches its own ray; a long ray is deferred once few
+// lanes are left marching (kSmallBudget, kCoopLanes).  Phase 2: the wave marches each
 // deferred ray 64 samples at a time, lane k taking sample k of the chunk (its
 // position by the same repeated additions t = t + step), and a ballot finds
 // the first negative sample.  Every sample position, value and decision is the
@@ -257,7 +260,8 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         int smalls = 0;
         for (; t < farv; t = t + step) {
             if (++guard > kMaxMarchSteps) break;
-            if (step == small && ++smalls > kSmallBudget) {
+            const int marching = __popcll(__ballot(1));  // lanes of the wave still in this loop
+            if (step == small && ++smalls > kSmallBudget && marching <= kCoopLanes) {
                 deferred = true;  // this sample (guard already counted) is the wave's first
                 break;
             }

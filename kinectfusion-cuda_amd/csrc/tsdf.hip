@@ -594,24 +594,33 @@ int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_
     return KF_OK;
 }
 
-int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
-                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events)
+float* volume_depth_max(kf_volume* v, int w, int h, hipStream_t s)
 {
     const size_t need = depth_max_floats(w, h);
     if (v->dmax_cap < need) {
         // grown outside the stream order: wait for work that may still read the old map
-        KF_HIP(hipStreamSynchronize(s));
+        if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
         (void)hipFree(v->dmax);
         v->dmax = nullptr;
         v->dmax_cap = 0;
-        KF_HIP(hipMalloc(&v->dmax, sizeof(float) * need));
+        if (hipMalloc(&v->dmax, sizeof(float) * need) != hipSuccess) return nullptr;
         v->dmax_cap = need;
     }
+    return v->dmax;
+}
+
+int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, const float* inv_pose,
+                const PoseState* dyn_state, hipStream_t s, hipEvent_t* list_events, bool dmax_ready)
+{
+    float* dmax = volume_depth_max(v, w, h, s);
+    if (!dmax) return fail(KF_ERR_OOM, "fuse: depth tile-max maps");
     FuseArgs a = fuse_args(depth, w, h, v, K, inv_pose, dyn_state);
-    a.dmax = v->dmax;
-    depth_max_kernel<<<dim3((unsigned)a.tx64, (unsigned)a.ty64), 256, 0, s>>>(depth, w, h, v->dmax, a.tx16, a.ty16,
-                                                                            a.tx64);
-    KF_LAUNCHED("depth_max_kernel");
+    a.dmax = dmax;
+    if (!dmax_ready) {
+        depth_max_kernel<<<dim3((unsigned)a.tx64, (unsigned)a.ty64), 256, 0, s>>>(depth, w, h, dmax, a.tx16, a.ty16,
+                                                                                a.tx64);
+        KF_LAUNCHED("depth_max_kernel");
+    }
     const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
     const int parity = (int)(v->fuse_launches & 1);
     fuse_cull_kernel<<<(unsigned)((nbricks + kCullThreads - 1) / kCullThreads), kCullThreads, 0, s>>>(
@@ -788,7 +797,7 @@ int kf_fuse(const float* depth_m, int w, int h, kf_volume* v, const kf_sensor* s
     hipStream_t s = kf::stream_of(stream);
     kf::CallTimer t(elapsed_ms, s);
     int st;
-    if ((st = t.start()) || (st = kf::launch_fuse(depth_m, w, h, v, sensor->intr[0], sensor->inv_pose, nullptr, s, nullptr)))
+    if ((st = t.start()) || (st = kf::launch_fuse(depth_m, w, h, v, sensor->intr[0], sensor->inv_pose, nullptr, s, nullptr, false)))
         return st;
     return t.finish();
 }
