@@ -213,6 +213,11 @@ typedef struct kf_pipeline_config {
     int voxel_format;           /* enum kf_voxel_format */
     float initial_pose[16];     /* camera -> world of the first frame */
     kf_icp_config icp;
+    /* optional visualisation volume (KinectFusion's m_voxel_grid_visualization,
+     * kinect_fusion.cpp:19,179-201): fused every frame like the tracking volume;
+     * vis_volume_n = 0 disables it (renders then use the tracking volume) */
+    int vis_volume_n;
+    float vis_volume_size, vis_mue;
 } kf_pipeline_config;
 typedef struct kf_pipeline kf_pipeline;
 enum kf_stage { KF_STAGE_PREPROCESS = 0, KF_STAGE_RAYCAST = 1, KF_STAGE_ICP = 2, KF_STAGE_FUSE = 3, KF_STAGE_COUNT = 4 };
@@ -234,9 +239,42 @@ int kf_pipeline_map(kf_pipeline* p, int which, int level, void** dev_ptr, int* w
 /* Per-stage event timing (kinect_fusion.cpp:216-224 prints these averages). */
 int kf_pipeline_set_timing(kf_pipeline* p, int enabled);
 int kf_pipeline_stage_times(kf_pipeline* p, float total_ms[KF_STAGE_COUNT], int64_t* frames);
+/* visualizeCurrentModel (kinect_fusion.cpp:185-201) without a window: raycasts level 0 of
+ * the visualisation volume (or the tracking volume) from pose (NULL = the current sensor pose)
+ * into maps of its own (not the tracking maps), then writes shading (shading = 1,
+ * display.cu:85-105) or the normal image (shading = 0, display.cu:64-83) as w x h RGBA8 pixels
+ * to rgba (device).  Waits for the frame work enqueued before it. */
+int kf_pipeline_render(kf_pipeline* p, const float pose[16], int shading, uint32_t* rgba);
 /* Average duration of the fuse's brick-list kernel (the voxel sweep) over the frames timed since
  * timing was enabled (ms), from events recorded around that kernel alone. */
 int kf_pipeline_fuse_kernel_ms(kf_pipeline* p, float* avg_ms, int64_t* launches);
+
+/* --------------------------------------------------------- visualisation --- */
+/* shadingToWindowContent (display.cu:85-105): per pixel min(|n . ray|, 1) in r, g, b with the
+ * level-0 ray of the sensor; RGBA8 (r in the low byte, alpha 255), bytes trunc(v * 255)
+ * clamped to [0, 255], NaN -> 0. */
+int kf_render_shading(const float* normal_map, int w, int h, const kf_sensor* sensor, uint32_t* rgba,
+                      kf_stream_t stream, float* elapsed_ms);
+/* normalMapToWindowContent (display.cu:64-83): normalize(inverse_rotation * n) as (r, g, b). */
+int kf_render_normals(const float* normal_map, int w, int h, const float inverse_rotation[9], uint32_t* rgba,
+                      kf_stream_t stream, float* elapsed_ms);
+
+/* -------------------------------------------------------- TUM sequences --- */
+/* RgbdDataset (rgbd_dataset.cpp:6-97) on the host: depth.txt and groundtruth.txt of a TUM
+ * RGB-D sequence; each depth frame is paired with the ground-truth pose nearest in time by the
+ * reference's forward search.  Depth images are 16-bit grayscale PNGs (TUM: 5000 codes per
+ * metre, so depth_scale = 1/5000). */
+typedef struct kf_tum kf_tum;
+int kf_tum_open(const char* dir, kf_tum** out);
+int kf_tum_close(kf_tum* s);
+int kf_tum_size(const kf_tum* s, int* frames, int* width, int* height);
+/* Frame i: depth (host, width*height codes; NULL to skip) and the ground-truth camera -> world
+ * pose (glm::mat4_cast of the quaternion with the translation, rgbd_dataset.cpp:67-69) of frame
+ * i + gt_offset.  gt_offset = 1 reproduces the reference, which reads the pose after advancing
+ * to the next image (kinect_fusion.cpp:165-166); 0 pairs each frame with its own pose. */
+int kf_tum_frame(kf_tum* s, int i, int gt_offset, uint16_t* depth, float pose[16]);
+/* poseError (general_helper.cpp:11-21): |R1 v - R2 v| for v = normalize(1, 1, 1), and |t1 - t2|. */
+int kf_pose_error(const float pose1[16], const float pose2[16], float* angle, float* distance);
 
 /* ---------------------------------------------------------------- batch ---
  * Many independent KinectFusion sessions (each its own volume, resolution,
@@ -292,6 +330,13 @@ int kf_batch_shape_get(kf_batch* b, kf_batch_shape* shape, int* tuned);
 /* GPU time of the ticks since the last call (events around each tick). */
 int kf_batch_tick_times(kf_batch* b, double* total_ms, int64_t* ticks, float* max_ms);
 int kf_batch_touched_total(kf_batch* b, unsigned long long* voxels);
+/* Session checkpoint (VoxelGrid::saveVoxelGrid/loadVoxelGrid, voxel_grid.cpp:43-80, plus the
+ * pose): path holds the volume in the reference's raw n^3 x (f, w) float format (any voxel
+ * format converts through it) and path + ".pose" the frame count and the camera -> world pose
+ * as text.  Restoring into a session of the same volume size resumes it: its next frame is
+ * tracked from the saved pose.  Both wait for the enqueued ticks. */
+int kf_batch_session_save(kf_batch* b, int session, const char* path);
+int kf_batch_session_restore(kf_batch* b, int session, const char* path);
 
 /* -------------------------------------------------------------- serving ---
  * A request loop over a kf_batch: sessions are admitted against an HBM budget
@@ -338,6 +383,9 @@ int kf_server_latencies(kf_server* s, double* out, size_t cap, size_t* count);
 int kf_server_state(kf_server* s, int ticket, int* state);
 int kf_server_pose(kf_server* s, int ticket, float pose[16]);
 int kf_server_volume(kf_server* s, int ticket, kf_volume** v);
+/* kf_batch_session_save / _restore for a resident session (restore before its first frame). */
+int kf_server_checkpoint(kf_server* s, int ticket, const char* path);
+int kf_server_restore(kf_server* s, int ticket, const char* path);
 
 /* A Poisson load: session k opens at opens_s[k] (seconds after the start) and
  * sends frames_per_session[k] frames with exponential inter-arrival times of

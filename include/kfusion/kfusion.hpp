@@ -410,6 +410,10 @@ struct KinectFusionConfig {
     float mue = 0.03f;
     int voxel_format = KF_VOXEL_F32;
     Mat4 initial_pose = identity4();
+    // the visualisation volume (kinect_fusion.h:32-33, 0 = none)
+    int visualization_volume_n = 0;
+    float visualization_volume_size = 4.0f;
+    float visualization_mue = 0.03f;
 };
 
 class KinectFusion {
@@ -430,6 +434,9 @@ public:
         c.mue = kf.mue;
         c.voxel_format = kf.voxel_format;
         for (int i = 0; i < 16; ++i) c.initial_pose[i] = kf.initial_pose[i];
+        c.vis_volume_n = kf.visualization_volume_n;
+        c.vis_volume_size = kf.visualization_volume_size;
+        c.vis_mue = kf.visualization_mue;
         c.icp.levels = (int)icp.iters_per_layer.size();
         for (int l = 0; l < c.icp.levels && l < 3; ++l) c.icp.iters_per_layer[l] = (int)icp.iters_per_layer[l];
         c.icp.width = (int)icp.width;
@@ -460,6 +467,12 @@ public:
         Mat4 p{};
         check(kf_pipeline_pose(m_p, p.data()), "getPose");
         return p;
+    }
+    // visualizeCurrentModel (kinect_fusion.cpp:185-201) into an RGBA8 device image of the frame size:
+    // shading (use_shading) or the normal image; from the current pose unless one is given
+    void visualize(uint32_t* rgba_device, bool shading, const Mat4* pose = nullptr)
+    {
+        check(kf_pipeline_render(m_p, pose ? pose->data() : nullptr, shading ? 1 : 0, rgba_device), "visualize");
     }
     kf_pipeline* handle() const { return m_p; }
 

@@ -13,7 +13,10 @@
 // optional autotuner times candidate shapes per stage on the first ticks
 // (direct launches with events between the stages) and keeps the fastest.
 #include <stddef.h>
+#include <stdio.h>
 #include <string.h>
+
+#include <string>
 
 #include <algorithm>
 #include <vector>
@@ -742,6 +745,64 @@ int kf_batch_tick_times(kf_batch* b, double* total_ms, int64_t* ticks, float* ma
     b->tick_ms_total = 0.0;
     b->ticks_timed = 0;
     b->tick_ms_max = 0.0f;
+    return KF_OK;
+}
+
+int kf_batch_session_save(kf_batch* b, int id, const char* path)
+{
+    Session* q = session_of(b, id);
+    if (!q || !path) return kf::fail(KF_ERR_ARG, "kf_batch_session_save: bad argument");
+    int st = kf_batch_sync(b);
+    if (st) return st;
+    float pose[32];  // pose, inverse pose (as the last fusion used it)
+    KF_HIP(hipMemcpy(pose, q->st->pose, sizeof(float) * 16, hipMemcpyDeviceToHost));
+    KF_HIP(hipMemcpy(pose + 16, q->st->inv_pose, sizeof(float) * 16, hipMemcpyDeviceToHost));
+    if ((st = kf_volume_save(q->vol, path))) return st;
+    const std::string pp = std::string(path) + ".pose";
+    FILE* fp = fopen(pp.c_str(), "w");
+    if (!fp) return kf::fail(KF_ERR_IO, "kf_batch_session_save: cannot open " + pp);
+    // 9 significant digits round-trip a float exactly
+    fprintf(fp, "frames %lld\npose", (long long)q->frames);
+    for (int k = 0; k < 16; ++k) fprintf(fp, " %.9g", pose[k]);
+    fprintf(fp, "\ninv_pose");
+    for (int k = 16; k < 32; ++k) fprintf(fp, " %.9g", pose[k]);
+    fprintf(fp, "\n");
+    const bool ok = fclose(fp) == 0;
+    return ok ? KF_OK : kf::fail(KF_ERR_IO, "kf_batch_session_save: write failed");
+}
+
+int kf_batch_session_restore(kf_batch* b, int id, const char* path)
+{
+    Session* q = session_of(b, id);
+    if (!q || !path) return kf::fail(KF_ERR_ARG, "kf_batch_session_restore: bad argument");
+    int st = kf_batch_sync(b);
+    if (st) return st;
+    const std::string pp = std::string(path) + ".pose";
+    FILE* fp = fopen(pp.c_str(), "r");
+    if (!fp) return kf::fail(KF_ERR_IO, "kf_batch_session_restore: cannot open " + pp);
+    long long frames = 0;
+    float pose[16], inv[16];
+    int got = fscanf(fp, " frames %lld pose", &frames);
+    for (int k = 0; k < 16 && got == 1; ++k) got = fscanf(fp, " %f", &pose[k]);
+    bool have_inv = got == 1 && fscanf(fp, " inv_pose") == 0;
+    for (int k = 0; k < 16 && have_inv; ++k) have_inv = fscanf(fp, " %f", &inv[k]) == 1;
+    fclose(fp);
+    if (got != 1 || frames < 0) return kf::fail(KF_ERR_IO, "kf_batch_session_restore: malformed " + pp);
+    if ((st = kf_volume_load(q->vol, path))) return st;
+    Pose32 p;
+    memcpy(p.m, pose, sizeof(pose));
+    if (have_inv) {
+        memcpy(p.m + 16, inv, sizeof(inv));
+    } else {
+        kf_sensor sensor;  // Sensor::setPose's inverse (sensor.h:30-34)
+        if ((st = kf_sensor_init(&sensor, q->cfg.fx, q->cfg.fy, q->cfg.cx, q->cfg.cy))) return st;
+        if ((st = kf_sensor_set_pose(&sensor, pose))) return st;
+        memcpy(p.m + 16, sensor.inv_pose, sizeof(sensor.inv_pose));
+    }
+    pose_state_init_kernel<<<1, 64, 0, b->s>>>(q->st, q->carry, p);
+    KF_LAUNCHED("pose_state_init_kernel");
+    KF_HIP(hipStreamSynchronize(b->s));
+    q->frames = frames;
     return KF_OK;
 }
 

@@ -182,4 +182,8 @@ int launch_icp_begin(kf_icp* icp, const float* pose16, hipStream_t s);
 int launch_icp_batch(const BatchHeader* hd, const IcpJob* jobs, const int* iters, int levels, int max_sessions,
                      int compensated, float stop_angle, float stop_dist, unsigned step_blocks, hipStream_t s);
 
+// ---- visualisation (render.hip)
+int launch_render(const float4* nmap, int w, int h, const float* mat9, int shading, uint32_t* out, hipStream_t s);
+void shading_matrix(const float* pose16, const float* kinv9, float* out9);
+
 }  // namespace kf

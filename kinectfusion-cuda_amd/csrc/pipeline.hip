@@ -15,6 +15,9 @@ struct kf_pipeline {
     hipStream_t s = nullptr;
     bool own_stream = false;
     kf_volume* vol = nullptr;
+    kf_volume* vis = nullptr;                     // optional visualisation volume
+    float4* rv = nullptr;                         // render maps (level 0), allocated on first render
+    float4* rn = nullptr;
     kf_icp* icp = nullptr;
     float* depth_m = nullptr;
     float* dpyr[3] = {nullptr, nullptr, nullptr};
@@ -73,6 +76,9 @@ int kf_pipeline_destroy(kf_pipeline* p)
     if (p->s) (void)hipStreamSynchronize(p->s);
     for (hipEvent_t e : p->pending) (void)hipEventDestroy(e);
     kf_volume_destroy(p->vol);
+    kf_volume_destroy(p->vis);
+    (void)hipFree(p->rv);
+    (void)hipFree(p->rn);
     kf_icp_destroy(p->icp);
     (void)hipFree(p->depth_m);
     for (int l = 0; l < 3; ++l) {
@@ -110,6 +116,8 @@ int kf_pipeline_create(const kf_pipeline_config* cfg, kf_stream_t stream, kf_pip
         p->own_stream = true;
     }
     if (st == KF_OK) st = kf_volume_create(cfg->volume_n, cfg->volume_size, cfg->mue, cfg->voxel_format, &p->vol);
+    if (st == KF_OK && cfg->vis_volume_n > 0)
+        st = kf_volume_create(cfg->vis_volume_n, cfg->vis_volume_size, cfg->vis_mue, cfg->voxel_format, &p->vis);
     if (st == KF_OK) st = kf_icp_create(&cfg->icp, &p->icp);
     kf_sensor sensor;
     if (st == KF_OK) st = kf_sensor_init(&sensor, cfg->fx, cfg->fy, cfg->cx, cfg->cy);
@@ -220,10 +228,12 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
     if ((st = kf::launch_fuse(p->depth_m, w, h, p->vol, p->K[0], nullptr, state, s, p->timing ? ev + 5 : nullptr,
                               fused)))
         return st;
-    if (p->timing) {
-        KF_HIP(hipEventRecord(ev[4], s));
+    if (p->timing) KF_HIP(hipEventRecord(ev[4], s));
+    // the visualisation volume takes the same depth at the same pose (kinect_fusion.cpp:179-183)
+    if (p->vis && (st = kf::launch_fuse(p->depth_m, w, h, p->vis, p->K[0], nullptr, state, s, nullptr, false)))
+        return st;
+    if (p->timing)
         for (auto e : ev) p->pending.push_back(e);
-    }
     p->frame += 1;
     return KF_OK;
 }
@@ -286,6 +296,41 @@ int kf_pipeline_map(kf_pipeline* p, int which, int level, void** dev_ptr, int* w
     *dev_ptr = ptr;
     if (w) *w = p->cfg.frame_width >> level;
     if (h) *h = p->cfg.frame_height >> level;
+    return KF_OK;
+}
+
+int kf_pipeline_render(kf_pipeline* p, const float pose[16], int shading, uint32_t* rgba)
+{
+    if (!p || !rgba) return kf::fail(KF_ERR_ARG, "kf_pipeline_render: null argument");
+    const int w = p->cfg.frame_width, h = p->cfg.frame_height;
+    const size_t px = (size_t)w * h;
+    if (!p->rv) {
+        KF_HIP(hipMalloc(&p->rv, sizeof(float4) * px));
+        KF_HIP(hipMalloc(&p->rn, sizeof(float4) * px));
+        std::vector<float4> inval(px, make_float4(0.0f, 0.0f, 0.0f, kf::kInvalid));
+        KF_HIP(hipMemcpy(p->rv, inval.data(), sizeof(float4) * px, hipMemcpyHostToDevice));
+        KF_HIP(hipMemcpy(p->rn, inval.data(), sizeof(float4) * px, hipMemcpyHostToDevice));
+    }
+    float P[16];
+    if (pose) {
+        memcpy(P, pose, sizeof(P));
+    } else {
+        KF_HIP(hipStreamSynchronize(p->s));
+        KF_HIP(hipMemcpy(P, p->icp->state->pose, sizeof(P), hipMemcpyDeviceToHost));
+    }
+    const kf_volume* v = p->vis ? p->vis : p->vol;
+    int st = kf::launch_raycast(v, P, nullptr, p->Ki[0], p->rv, p->rn, w, h, p->s);
+    if (st) return st;
+    float m[9];
+    if (shading) {
+        kf::shading_matrix(P, p->Ki[0], m);
+    } else {
+        // glm::mat3(inverse pose) of a rigid pose: R^T
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) m[c * 3 + r] = P[r * 4 + c];
+    }
+    if ((st = kf::launch_render(p->rn, w, h, m, shading, rgba, p->s))) return st;
+    KF_HIP(hipStreamSynchronize(p->s));
     return KF_OK;
 }
 

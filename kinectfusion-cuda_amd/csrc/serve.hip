@@ -392,6 +392,22 @@ int kf_server_volume(kf_server* s, int id, kf_volume** v)
     return kf_batch_volume(s->b, q->sid, v);
 }
 
+int kf_server_checkpoint(kf_server* s, int id, const char* path)
+{
+    Ticket* q = ticket_of(s, id);
+    if (!q || q->state != 1) return kf::fail(KF_ERR_ARG, "kf_server_checkpoint: the session is not resident");
+    return kf_batch_session_save(s->b, q->sid, path);
+}
+
+int kf_server_restore(kf_server* s, int id, const char* path)
+{
+    Ticket* q = ticket_of(s, id);
+    if (!q || q->state != 1) return kf::fail(KF_ERR_ARG, "kf_server_restore: the session is not resident");
+    if (q->count > 0 || q->inflight > 0)
+        return kf::fail(KF_ERR_ARG, "kf_server_restore: restore before the session's first frame");
+    return kf_batch_session_restore(s->b, q->sid, path);
+}
+
 int kf_server_run_load(kf_server* s, const kf_poisson_load* L, int* tickets_out, kf_server_stats* out)
 {
     if (!s || !L || L->sessions <= 0 || !L->configs || !L->banks || !L->bank_len || !L->frames_per_session ||

@@ -22,7 +22,7 @@ from ._capi import (KF_ERR_ARG, KF_ERR_BUDGET, KF_ERR_DIMS, KF_ERR_IO, KF_ERR_NO
 __all__ = [
     "KfError", "lib", "init", "DeviceArray", "GridMap", "DepthMap", "GridMapPyramid", "Sensor",
     "RigidTransform3D", "VoxelGrid", "ICP", "IcpConfig", "LinearLeastSquares", "kernel", "Pipeline",
-    "Batch", "SessionConfig", "session_bytes", "Server", "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
+    "Batch", "SessionConfig", "session_bytes", "Server", "TumSequence", "pose_error", "KF_VOXEL_F32", "KF_VOXEL_SNORM16_U8", "STAGE_NAMES",
 ]
 
 
@@ -418,7 +418,7 @@ class Pipeline:
 
     def __init__(self, width=640, height=480, fx=525.0, fy=525.0, cx=319.5, cy=239.5, depth_scale=1e-3,
                  volume_n=256, volume_size=3.0, mue=0.03, voxel_format=KF_VOXEL_F32, initial_pose=None,
-                 icp=None, unmirror=False):
+                 icp=None, unmirror=False, vis_volume=None):
         init()
         c = kf_pipeline_config()
         c.frame_width, c.frame_height = width, height
@@ -431,6 +431,8 @@ class Pipeline:
         for i in range(16):
             c.initial_pose[i] = float(pose[i])
         c.icp = icp if icp is not None else IcpConfig(width=width, height=height)
+        if vis_volume:  # (n, size, mue) of the visualisation volume
+            c.vis_volume_n, c.vis_volume_size, c.vis_mue = vis_volume
         self.cfg = c
         h = C.c_void_p()
         check(lib().kf_pipeline_create(C.byref(c), None, C.byref(h)), "kf_pipeline_create")
@@ -457,6 +459,15 @@ class Pipeline:
         shape = (h.value, w.value) if ch == 1 else (h.value, w.value, ch)
         out = np.empty(shape, np.float32)
         check(lib().kf_memcpy_d2h(_p(out), ptr, out.nbytes, None), "map download")
+        return out
+
+    def render(self, shading=True, pose=None):
+        """RGBA8 image (h, w) of the model from pose (default: the current pose)."""
+        img = DeviceArray((self.height, self.width), np.uint32)
+        pp = None if pose is None else _p(np.ascontiguousarray(pose, np.float32))
+        check(lib().kf_pipeline_render(self.h, pp, int(shading), img.ptr), "kf_pipeline_render")
+        out = img.download()
+        img.free()
         return out
 
     def volume_download(self):
@@ -591,6 +602,12 @@ class Batch:
         check(lib().kf_volume_download(v, _p(fw)), "kf_volume_download")
         return fw[0::2].copy(), fw[1::2].copy()
 
+    def save_session(self, sid, path):
+        check(lib().kf_batch_session_save(self.h, sid, str(path).encode()), "kf_batch_session_save")
+
+    def restore_session(self, sid, path):
+        check(lib().kf_batch_session_restore(self.h, sid, str(path).encode()), "kf_batch_session_restore")
+
     def shape(self):
         sh = kf_batch_shape()
         tuned = C.c_int()
@@ -685,6 +702,12 @@ class Server:
         check(lib().kf_volume_download(v, _p(fw)), "kf_volume_download")
         return fw[0::2].copy(), fw[1::2].copy()
 
+    def checkpoint(self, ticket, path):
+        check(lib().kf_server_checkpoint(self.h, ticket, str(path).encode()), "kf_server_checkpoint")
+
+    def restore(self, ticket, path):
+        check(lib().kf_server_restore(self.h, ticket, str(path).encode()), "kf_server_restore")
+
     def stats(self):
         st = kf_server_stats()
         check(lib().kf_server_stats_get(self.h, C.byref(st)), "kf_server_stats_get")
@@ -734,3 +757,47 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class TumSequence:
+    """A TUM RGB-D sequence (kf_tum_*, rgbd_dataset.cpp:6-97): frame(i) -> (uint16 depth, pose).
+    gt_offset=1 reproduces the reference's pairing of frame i with the pose of frame i + 1."""
+
+    def __init__(self, path, gt_offset=0):
+        h = C.c_void_p()
+        check(lib().kf_tum_open(str(path).encode(), C.byref(h)), "kf_tum_open")
+        self.h = h
+        self.gt_offset = gt_offset
+        n, w, hh = C.c_int(), C.c_int(), C.c_int()
+        check(lib().kf_tum_size(h, C.byref(n), C.byref(w), C.byref(hh)), "kf_tum_size")
+        self.n, self.width, self.height = n.value, w.value, hh.value
+
+    def __len__(self):
+        return self.n
+
+    def frame(self, i):
+        depth = np.zeros((self.height, self.width), np.uint16)
+        pose = np.zeros(16, np.float32)
+        check(lib().kf_tum_frame(self.h, i, self.gt_offset, _p(depth), _p(pose)), "kf_tum_frame")
+        return depth, pose
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().kf_tum_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pose_error(pose1, pose2):
+    """poseError (general_helper.cpp:11-21) of two column-major camera -> world poses."""
+    a, d = C.c_float(), C.c_float()
+    p1 = np.ascontiguousarray(pose1, np.float32).reshape(-1)
+    p2 = np.ascontiguousarray(pose2, np.float32).reshape(-1)
+    check(lib().kf_pose_error(_p(p1), _p(p2), C.byref(a), C.byref(d)), "kf_pose_error")
+    return a.value, d.value
+

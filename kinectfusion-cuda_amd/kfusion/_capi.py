@@ -69,6 +69,9 @@ class kf_pipeline_config(C.Structure):
         ("voxel_format", C.c_int),
         ("initial_pose", C.c_float * 16),
         ("icp", kf_icp_config),
+        ("vis_volume_n", C.c_int),
+        ("vis_volume_size", C.c_float),
+        ("vis_mue", C.c_float),
     ]
 
 
@@ -215,6 +218,14 @@ SIGNATURES = {
     "kf_pipeline_set_timing": (I, [P, I]),
     "kf_pipeline_stage_times": (I, [P, P, C.POINTER(I64)]),
     "kf_pipeline_fuse_kernel_ms": (I, [P, FP, C.POINTER(I64)]),
+    "kf_pipeline_render": (I, [P, P, I, P]),
+    "kf_render_shading": (I, [P, I, I, C.POINTER(kf_sensor), P, P, FP]),
+    "kf_render_normals": (I, [P, I, I, P, P, P, FP]),
+    "kf_tum_open": (I, [C.c_char_p, C.POINTER(P)]),
+    "kf_tum_close": (I, [P]),
+    "kf_tum_size": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "kf_tum_frame": (I, [P, I, I, P, P]),
+    "kf_pose_error": (I, [P, P, FP, FP]),
     "kf_session_bytes": (I, [C.POINTER(kf_session_config), C.POINTER(SZ)]),
     "kf_batch_create": (I, [C.POINTER(kf_batch_config), P, C.POINTER(P)]),
     "kf_batch_destroy": (I, [P]),
@@ -229,6 +240,10 @@ SIGNATURES = {
     "kf_batch_shape_get": (I, [P, C.POINTER(kf_batch_shape), C.POINTER(I)]),
     "kf_batch_tick_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64), FP]),
     "kf_batch_touched_total": (I, [P, C.POINTER(C.c_ulonglong)]),
+    "kf_batch_session_save": (I, [P, I, C.c_char_p]),
+    "kf_batch_session_restore": (I, [P, I, C.c_char_p]),
+    "kf_server_checkpoint": (I, [P, I, C.c_char_p]),
+    "kf_server_restore": (I, [P, I, C.c_char_p]),
     "kf_server_create": (I, [C.POINTER(kf_server_config), C.POINTER(P)]),
     "kf_server_destroy": (I, [P]),
     "kf_server_now": (C.c_double, []),

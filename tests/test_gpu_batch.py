@@ -159,3 +159,31 @@ def test_session_admission_and_errors(kf, inputs):
         kf.session_bytes(bad)
     assert kf.session_bytes(session_cfg(kf, SESSIONS[0], seq)) > 128 ** 3 * 8
     b.close()
+
+
+def test_checkpoint_restore_resumes_bit_exact(kf, inputs, tmp_path):
+    """Session checkpoint (raw volume + pose, voxel_grid.cpp:43-80): saving after 3 ticks and
+    resuming in a new batch gives the uninterrupted run's poses, volumes and maps."""
+    specs, ins = SESSIONS[:2], inputs[:2]
+    whole = run_batch(kf, specs, ins, lambda k, t: t, use_graph=True)
+    b = kf.Batch(2, icp=icp_cfg(kf))
+    ids = [b.add(session_cfg(kf, sp, ins[k][0])) for k, sp in enumerate(specs)]
+    for t in range(3):
+        announce(f"checkpoint run tick {t}")
+        b.tick(ids, [ins[k][2].ptr.value + t * sp["w"] * sp["h"] * 2 for k, sp in enumerate(specs)])
+    for k in range(2):
+        b.save_session(ids[k], tmp_path / f"s{k}.bin")
+    b.close()
+    b = kf.Batch(2, icp=icp_cfg(kf))
+    ids = [b.add(session_cfg(kf, sp, ins[k][0])) for k, sp in enumerate(specs)]
+    for k in range(2):
+        b.restore_session(ids[k], tmp_path / f"s{k}.bin")
+    for t in range(3, TICKS):
+        announce(f"resumed run tick {t}")
+        b.tick(ids, [ins[k][2].ptr.value + t * sp["w"] * sp["h"] * 2 for k, sp in enumerate(specs)])
+    b.sync()
+    for k in range(2):
+        maps = [b.map(ids[k], which, l) for which in (2, 4, 5) for l in range(3)] + [b.map(ids[k], 0)]
+        got = dict(pose=b.pose(ids[k]), vol=b.volume_download(ids[k]), maps=maps)
+        assert_same(got, whole[k], f"session {k} resumed")
+    b.close()
