@@ -109,7 +109,7 @@ struct kf_batch {
     int next = 0;
     unsigned* fuse_count = nullptr;
     unsigned* ray_next = nullptr;  // 16 raycast tile counters
-    unsigned* fuse_list = nullptr;
+    unsigned long long* fuse_list = nullptr;
     size_t list_cap = 0;
     unsigned long long* touched = nullptr;
     kf_batch_shape shape{};
@@ -198,7 +198,7 @@ int ensure_list(kf_batch* b)
     (void)hipFree(b->fuse_list);
     b->fuse_list = nullptr;
     b->list_cap = 0;
-    int st = alloc((void**)&b->fuse_list, sizeof(unsigned) * need, nullptr);
+    int st = alloc((void**)&b->fuse_list, sizeof(unsigned long long) * need, nullptr);
     if (st == KF_OK) b->list_cap = need;
     return st;
 }

@@ -46,15 +46,19 @@ struct PixelIn {
     v3 nl;       // live normal
     bool ok;     // vertex and normal valid
 };
-__device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
+__device__ __forceinline__ PixelIn icp_pixel_from(const NormalStencil& ns)
 {
     PixelIn in;
-    in.ok = map_valid(a.V, a.w, a.h, u, v);
-    in.vc = in.ok ? xyz(a.V[(int64_t)v * a.w + u]) : mk3(0.0f, 0.0f, 0.0f);
-    float4 nl = compute_normal(a.V, a.w, a.h, u, v);
+    in.ok = ns.c.w == kValid;
+    in.vc = in.ok ? xyz(ns.c) : mk3(0.0f, 0.0f, 0.0f);
+    float4 nl = normal_of(ns);
     in.nl = xyz(nl);
     in.ok = in.ok && nl.w != kInvalid;
     return in;
+}
+__device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
+{
+    return icp_pixel_from(load_normal_stencil(a.V, a.w, a.h, u, v));
 }
 
 // false = zero row (rejection reasons as icp.cu:29-70, evaluated in an order
@@ -132,34 +136,80 @@ __device__ __forceinline__ void accumulate(float* s, float* c, const float* row,
     }
 }
 
-// Reduces the lane values of a workgroup and writes one partial row.
+// One halving step of the lane reduction: the first n/2 entries of s (and c)
+// become, on the lane with bit `off` clear, its first half plus the partner's
+// copy and, on the other lane, its second half plus the partner's copy.  The
+// operand of the lower lane goes first, which is the order in which lane 0 of
+// a butterfly (xor off) adds, so the sums equal the butterfly's bit for bit
+// with about a sixth of the lane exchanges.
+template <bool COMP, int N>
+__device__ __forceinline__ void halve_lanes(float* s, float* c, int off, bool upper)
+{
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        const float send = upper ? s[i] : s[N / 2 + i];
+        const float keep = upper ? s[N / 2 + i] : s[i];
+        const float recv = __shfl_xor(send, off, 64);
+        if (COMP) {
+            const float csend = upper ? c[i] : c[N / 2 + i];
+            const float ckeep = upper ? c[N / 2 + i] : c[i];
+            const float crecv = __shfl_xor(csend, off, 64);
+            float ls = upper ? recv : keep, lc = upper ? crecv : ckeep;
+            comp_merge(ls, lc, upper ? keep : recv, upper ? ckeep : crecv);
+            s[i] = ls;
+            c[i] = lc;
+        } else {
+            s[i] = keep + recv;
+        }
+    }
+}
+
+// Reduces the lane values of a workgroup and writes one partial row.  The 27
+// terms are halved across the lanes (28 -> 14 -> 7 -> 4 -> 2 -> 1, one zero
+// pad at 27 and one at the 7 -> 8 step): lane L ends with term
+// 14*b5 + 7*b4 + 4*b3 + 2*b2 + b1 of its bits, every pair (L, L^1) holding it.
 template <bool COMP>
-__device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, float* partials, int stride, int row)
+__device__ __forceinline__ void block_reduce_store(float* s_in, float* c_in, int cnt, float* partials, int stride,
+                                                   int row)
 {
     __shared__ float red[kThreads / 64][2 * kTerms + 1];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < kTerms; ++q) {
-            float so = __shfl_xor(s[q], off, 64);
-            if (COMP) {
-                float co = __shfl_xor(c[q], off, 64);
-                comp_merge(s[q], c[q], so, co);
-            } else {
-                s[q] = s[q] + so;
-            }
-        }
-        cnt += __shfl_xor(cnt, off, 64);
-    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
+    float s[28], c[28];
 #pragma unroll
-        for (int q = 0; q < kTerms; ++q) {
-            red[wave][q] = s[q];
-            red[wave][kTerms + q] = COMP ? c[q] : 0.0f;
-        }
-        red[wave][2 * kTerms] = (float)cnt;
+    for (int q = 0; q < kTerms; ++q) {
+        s[q] = s_in[q];
+        c[q] = COMP ? c_in[q] : 0.0f;
     }
+    s[27] = c[27] = 0.0f;
+    halve_lanes<COMP, 28>(s, c, 32, (lane & 32) != 0);
+    halve_lanes<COMP, 14>(s, c, 16, (lane & 16) != 0);
+    s[7] = c[7] = 0.0f;
+    halve_lanes<COMP, 8>(s, c, 8, (lane & 8) != 0);
+    halve_lanes<COMP, 4>(s, c, 4, (lane & 4) != 0);
+    halve_lanes<COMP, 2>(s, c, 2, (lane & 2) != 0);
+    {
+        // last step: both lanes of the pair add the same two operands, lower first
+        const bool upper = (lane & 1) != 0;
+        const float recv = __shfl_xor(s[0], 1, 64);
+        if (COMP) {
+            const float crecv = __shfl_xor(c[0], 1, 64);
+            float ls = upper ? recv : s[0], lc = upper ? crecv : c[0];
+            comp_merge(ls, lc, upper ? s[0] : recv, upper ? c[0] : crecv);
+            s[0] = ls;
+            c[0] = lc;
+        } else {
+            s[0] = s[0] + recv;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    const int sub = 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + ((lane >> 1) & 1);
+    const int q = 14 * ((lane >> 5) & 1) + 7 * ((lane >> 4) & 1) + sub;
+    if ((lane & 1) == 0 && sub < 7 && q < kTerms) {
+        red[wave][q] = s[0];
+        red[wave][kTerms + q] = COMP ? c[0] : 0.0f;
+    }
+    if (lane == 0) red[wave][2 * kTerms] = (float)cnt;
     __syncthreads();
     if (threadIdx.x < kTerms) {
         const int q = threadIdx.x;
@@ -180,19 +230,33 @@ __device__ __forceinline__ void block_reduce_store(float* s, float* c, int cnt, 
 // The pixels of lane (block, k, lane) and their pose-independent inputs.
 struct LanePixels {
     PixelIn in[kPixPerThread];
-    bool live[kPixPerThread];
 };
-__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp, int block)
+// Their vertex loads only (a pixel past the map reads as invalid), so the
+// loads can be in flight while the workgroup finishes the previous iteration.
+struct LaneLoads {
+    NormalStencil ns[kPixPerThread];
+};
+__device__ __forceinline__ void issue_lane_loads(const IcpArgs& a, LaneLoads& ll, int block)
 {
     const int64_t npx = (int64_t)a.w * a.h;
 #pragma unroll
     for (int k = 0; k < kPixPerThread; ++k) {
         int64_t px = ((int64_t)block * kPixPerThread + k) * kThreads + threadIdx.x;
-        lp.live[k] = px < npx;
-        int u = lp.live[k] ? (int)(px % a.w) : 0, v = lp.live[k] ? (int)(px / a.w) : 0;
-        lp.in[k] = icp_pixel_in(a, u, v);
-        lp.in[k].ok = lp.in[k].ok && lp.live[k];
+        const bool live = px < npx;
+        int u = live ? (int)(px % a.w) : -1, v = live ? (int)(px / a.w) : -1;
+        ll.ns[k] = load_normal_stencil(a.V, a.w, a.h, u, v);
     }
+}
+__device__ __forceinline__ void finish_lane_pixels(const LaneLoads& ll, LanePixels& lp)
+{
+#pragma unroll
+    for (int k = 0; k < kPixPerThread; ++k) lp.in[k] = icp_pixel_from(ll.ns[k]);
+}
+__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp, int block)
+{
+    LaneLoads ll;
+    issue_lane_loads(a, ll, block);
+    finish_lane_pixels(ll, lp);
 }
 
 // pose: R (9), t (3), Rp (9), tp (3) of the iteration, in LDS.  Writes partial
@@ -439,33 +503,67 @@ __device__ void pose_error(const float* R1, const float* t1, const float* R2, co
     out[1] = len3(sub3(mk3(t2[0], t2[1], t2[2]), mk3(t1[0], t1[1], t1[2])));
 }
 
-// Sums the partial rows (fixed order) into 27 doubles + count in LDS.
+// Sums the partial rows (fixed order) into 27 doubles + count in LDS.  Wave w
+// owns rows q = w, w + 4, ... (7 of the 28); lane l adds blocks l, l + 64, ...
+// in that order, then the lanes combine by butterfly.  The loads of all of a
+// wave's rows for two 64-block chunks are issued before any is added, so a
+// row set costs one memory round trip per 128 blocks.
 template <bool COMP>
 __device__ void sum_partials(const float* partials, int stride, int nblocks, double* tot, double* count)
 {
+    constexpr int kRows = (kTerms + 1 + 3) / 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int q = wave; q <= kTerms; q += kThreads / 64) {
-        const bool is_count = (q == kTerms);
-        const float* sp = partials + (int64_t)(is_count ? kCountSlot : q) * stride;
-        const float* cp = partials + (int64_t)(kTerms + q) * stride;
-        float s = 0.0f, c = 0.0f;
-        for (int b = lane; b < nblocks; b += 64) {
-            if (COMP && !is_count) comp_merge(s, c, sp[b], cp[b]);
-            else s = s + sp[b];
+    float s[kRows], c[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) s[k] = c[k] = 0.0f;
+    for (int b0 = 0; b0 < nblocks; b0 += 128) {
+        float vs[2][kRows], vc[2][kRows];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int b = b0 + 64 * h + lane;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                const int q = wave + 4 * k;
+                const bool is_count = (q == kTerms);
+                vs[h][k] = vc[h][k] = 0.0f;
+                if (q <= kTerms && b < nblocks) {
+                    vs[h][k] = partials[(int64_t)(is_count ? kCountSlot : q) * stride + b];
+                    if (COMP && !is_count) vc[h][k] = partials[(int64_t)(kTerms + q) * stride + b];
+                }
+            }
         }
 #pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int b = b0 + 64 * h + lane;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                const int q = wave + 4 * k;
+                if (q <= kTerms && b < nblocks) {
+                    if (COMP && q != kTerms) comp_merge(s[k], c[k], vs[h][k], vc[h][k]);
+                    else s[k] = s[k] + vs[h][k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const int q = wave + 4 * k;
+        if (q > kTerms) continue;  // wave-uniform
+        const bool is_count = (q == kTerms);
+        float sk = s[k], ck = c[k];
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-            float so = __shfl_xor(s, off, 64);
+            float so = __shfl_xor(sk, off, 64);
             if (COMP && !is_count) {
-                float co = __shfl_xor(c, off, 64);
-                comp_merge(s, c, so, co);
+                float co = __shfl_xor(ck, off, 64);
+                comp_merge(sk, ck, so, co);
             } else {
-                s = s + so;
+                sk = sk + so;
             }
         }
         if (lane == 0) {
-            if (is_count) *count = (double)s;
-            else tot[q] = COMP ? (double)s + (double)c : (double)s;
+            if (is_count) *count = (double)sk;
+            else tot[q] = COMP ? (double)sk + (double)ck : (double)sk;
         }
     }
     __syncthreads();
@@ -635,14 +733,16 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
     __shared__ int run;
     __shared__ double tot[kTerms];
     __shared__ double count;
-    // phase A: pose-independent loads of this lane's pixels
-    LanePixels lp;
-    load_lane_pixels(a, lp, blockIdx.x);
+    // phase A: pose-independent loads of this lane's pixels (in flight during B)
+    LaneLoads ll;
+    issue_lane_loads(a, ll, blockIdx.x);
     // phase B: finish the previous iteration
     begin_iteration<COMP>(a.st, cin, cout, pin, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle, stop_dist,
                           pose, &run, tot, &count);
     if (!run) return;
-    // phase C: association + reduction of this iteration
+    // phase C: live normals, association + reduction of this iteration
+    LanePixels lp;
+    finish_lane_pixels(ll, lp);
     associate_reduce<COMP>(a, lp, pose, blockIdx.x);
 }
 
@@ -869,6 +969,23 @@ int launch_icp_begin(kf_icp* icp, const float* pose16, hipStream_t s)
     return KF_OK;
 }
 
+static IcpArgs level_args(const kf_icp* icp, const IcpLevel& L, int l)
+{
+    IcpArgs a{};
+    a.V = L.V;
+    a.TV = L.TV;
+    a.TN = L.TN;
+    a.w = L.w;
+    a.h = L.h;
+    memcpy(a.K, L.K, sizeof(a.K));
+    a.dist_thresh = icp->cfg.distance_thresh;
+    a.cos_thresh = icp->cos_thresh;
+    a.st = icp->state;
+    a.level = l;
+    a.stride = blocks_for((int64_t)L.w * L.h);
+    return a;
+}
+
 // Enqueues the iteration launches; `events` (optional) receives one event
 // after every launch.  Returns the number of launches in *launches.
 static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s, hipEvent_t* events,
@@ -878,21 +995,9 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
     int k = 0;
     for (int l = n_levels - 1; l >= 0; --l)
         for (int it = 0; it < icp->cfg.iters_per_layer[l]; ++it, ++k) {
-            const IcpLevel& L = levels[l];
-            IcpArgs a{};
-            a.V = L.V;
-            a.TV = L.TV;
-            a.TN = L.TN;
-            a.w = L.w;
-            a.h = L.h;
-            memcpy(a.K, L.K, sizeof(a.K));
-            a.dist_thresh = icp->cfg.distance_thresh;
-            a.cos_thresh = icp->cos_thresh;
-            a.st = icp->state;
-            a.level = l;
+            IcpArgs a = level_args(icp, levels[l], l);
             a.partials = partials_of(icp, k);
-            const int blocks = blocks_for((int64_t)L.w * L.h);
-            a.stride = blocks;
+            const int blocks = a.stride;
             if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
             if (icp->cfg.compensated)
                 icp_step_kernel<true><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),

@@ -126,16 +126,41 @@ __device__ __forceinline__ float kf_expf(float x)
 }
 
 // Live normal at (i,j) of a vertex map (device_helper.cuh:81-110).
-__device__ __forceinline__ float4 compute_normal(const float4* v, int w, int h, int i, int j)
+// The three vertices a normal needs: the pixel, its right and its lower
+// neighbour (out-of-range entries read as invalid).  The three loads are
+// independent, so they are in flight together; an out-of-range entry loads
+// element 0 and is then replaced (a value select, not an address select).
+struct NormalStencil {
+    float4 c, r, d;
+};
+__device__ __forceinline__ float4 load_or_invalid(const float4* v, bool ok, int64_t q)
 {
-    if (map_valid(v, w, h, i, j) && map_valid(v, w, h, i + 1, j) && map_valid(v, w, h, i, j + 1)) {
-        v3 c = xyz(v[(int64_t)j * w + i]);
-        v3 r = xyz(v[(int64_t)j * w + i + 1]);
-        v3 d = xyz(v[(int64_t)(j + 1) * w + i]);
+    float4 e = v[ok ? q : 0];
+    if (!ok) e.w = kInvalid;
+    return e;
+}
+__device__ __forceinline__ NormalStencil load_normal_stencil(const float4* v, int w, int h, int i, int j)
+{
+    const bool ic = i >= 0 && j >= 0 && i < w && j < h;
+    const int64_t q = (int64_t)j * w + i;
+    NormalStencil s;
+    s.c = load_or_invalid(v, ic, q);
+    s.r = load_or_invalid(v, ic && i + 1 < w, q + 1);
+    s.d = load_or_invalid(v, ic && j + 1 < h, q + w);
+    return s;
+}
+__device__ __forceinline__ float4 normal_of(const NormalStencil& s)
+{
+    if (s.c.w == kValid && s.r.w == kValid && s.d.w == kValid) {
+        v3 c = xyz(s.c), r = xyz(s.r), d = xyz(s.d);
         v3 n = normalize3(cross3(sub3(r, c), sub3(d, c)));
         return make_float4(n.x, n.y, n.z, kValid);
     }
     return make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+}
+__device__ __forceinline__ float4 compute_normal(const float4* v, int w, int h, int i, int j)
+{
+    return normal_of(load_normal_stencil(v, w, h, i, j));
 }
 
 // ---------------------------------------------------------------------------

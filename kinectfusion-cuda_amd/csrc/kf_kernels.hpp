@@ -27,7 +27,7 @@ struct BatchHeader {
     int n_fuse;                       // fuse jobs
     unsigned fuse_bricks;             // bricks over all fuse jobs
     unsigned* fuse_count;
-    unsigned* fuse_list;
+    unsigned long long* fuse_list;    // job << 32 | brick of the job's volume
     unsigned long long* touched;
     unsigned* ray_next;  // 16 tile counters (8 XCD bands x 2 voxel formats), cleared by the first kernel
 };

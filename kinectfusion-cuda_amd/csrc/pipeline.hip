@@ -163,7 +163,9 @@ int kf_pipeline_frame(kf_pipeline* p, const uint16_t* raw)
     hipStream_t s = p->s;
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (p->timing) {
-        for (auto& e : ev) KF_HIP(hipEventCreate(&e));
+        // timing-only events: no system-scope cache write-back when they complete
+        // (their timestamps are read after a stream synchronisation)
+        for (auto& e : ev) KF_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         KF_HIP(hipEventRecord(ev[0], s));
     }
     int st;

@@ -182,7 +182,7 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
 // ray in 40 grazes a surface and takes hundreds of small steps; while many
 // lanes march, the wave is busy anyway).
 constexpr int kSmallBudget = 32;
-constexpr int kCoopLanes = 8;
+constexpr int kCoopLanes = 16;
 
 // One trilinear evaluation of the march at t: 1 = skipped (the sample is known
 // to be >= 0.99), 0 = evaluated into f.

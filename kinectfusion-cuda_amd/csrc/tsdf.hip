@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "kf_device.hpp"
@@ -105,14 +106,23 @@ __global__ __launch_bounds__(256) void depth_max_kernel(const float* __restrict_
 }
 
 // tsdf.cu:20-75 for the 8 voxels of one brick column (lane = (j, i) of the
-// brick, kl = 0..7 along z).  The projection of all 8 voxels is computed first
-// and their depth and voxel loads are issued together, so a lane has up to 8
-// independent memory requests in flight.  Returns the number of updated voxels.
+// brick, kl = 0..7 along z).  The projection of all 8 voxels is computed first;
+// then, for every voxel that projects into the image, the depth gather and the
+// voxel's own f and w loads are issued together (the voxel loads do not wait
+// for the depth test, so a brick costs one memory round trip, not two).  A
+// voxel whose depth test fails is read but not written.  Returns the number of
+// updated voxels.
 template <int FMT>
 __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v3 col, const float* zk, int64_t base,
                                            float& fmin)
 {
+    typedef typename std::conditional<FMT == KF_VOXEL_F32, float, int16_t>::type FT;
+    typedef typename std::conditional<FMT == KF_VOXEL_F32, float, uint8_t>::type WT;
+    FT* F = static_cast<FT*>(a.f);
+    WT* Wt = static_cast<WT*>(a.wt);
     float dep[8], pz[8];
+    FT fv[8];
+    WT wv[8];
     int pix[8];
     bool hit[8];
 #pragma unroll
@@ -134,7 +144,16 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         pz[kl] = pe.z;
     }
 #pragma unroll
-    for (int kl = 0; kl < 8; ++kl) dep[kl] = hit[kl] ? a.depth[pix[kl]] : 0.0f;
+    for (int kl = 0; kl < 8; ++kl) {
+        dep[kl] = 0.0f;
+        fv[kl] = FT(0);
+        wv[kl] = WT(0);
+        if (hit[kl]) {
+            dep[kl] = a.depth[pix[kl]];
+            fv[kl] = F[base + 64 * kl];
+            wv[kl] = Wt[base + 64 * kl];
+        }
+    }
     int cnt = 0;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
@@ -145,25 +164,21 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         f = (f < 1.0f) ? f : 1.0f;
         const int64_t q = base + 64 * kl;
         if (FMT == KF_VOXEL_F32) {
-            float* F = static_cast<float*>(a.f);
-            float* Wt = static_cast<float*>(a.wt);
-            float wt = 1.0f / d;
-            float Fv = F[q], Wv = Wt[q];
+            const float wt = 1.0f / d;
+            float Fv = (float)fv[kl], Wv = (float)wv[kl];
             Fv = (Fv * Wv + f * wt) / (Wv + wt);
             Wv = Wv + wt;
-            F[q] = Fv;
-            Wt[q] = Wv;
+            F[q] = (FT)Fv;
+            Wt[q] = (WT)Wv;
             fmin = fminf(fmin, Fv);
         } else {
             // precision form: unit weight per observation, count capped at 128
-            int16_t* F = static_cast<int16_t*>(a.f);
-            uint8_t* Wt = static_cast<uint8_t*>(a.wt);
-            float Wv = (float)Wt[q];
-            float Fv = snorm_decode(F[q]);
+            float Wv = (float)wv[kl];
+            float Fv = snorm_decode((int16_t)fv[kl]);
             Fv = (Fv * Wv + f) / (Wv + 1.0f);
             const int16_t qf = snorm_encode(Fv);
-            F[q] = qf;
-            Wt[q] = (uint8_t)(Wv < 128.0f ? Wv + 1.0f : 128.0f);
+            F[q] = (FT)qf;
+            Wt[q] = (WT)(Wv < 128.0f ? Wv + 1.0f : 128.0f);
             fmin = fminf(fmin, snorm_decode(qf));
         }
         ++cnt;
@@ -222,20 +237,32 @@ constexpr int kFuseWaves = 4;
 // One wave fuses brick `brick` (lane = (j, i) inside the brick, 8 voxels along
 // z) and lowers the brick's and its super-brick's bound of F.  Returns the
 // lane's count of updated voxels.
+//
+// The brick's z table entries and its bound are uniform: they are read through
+// the constant address space (scalar loads, issued together at the start; the
+// bound is only written by this wave, after the read).
+typedef __attribute__((address_space(4))) const float ConstFloat;
+typedef __attribute__((address_space(4))) const int ConstInt;
+typedef __attribute__((address_space(4))) const unsigned int ConstUint;
+
 template <int FMT>
 __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, int64_t brick, int lane)
 {
     const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
     const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+    float zk[8];
+#pragma unroll
+    for (int kl = 0; kl < 8; ++kl) zk[kl] = ((ConstFloat*)a.zk)[bk * 8 + kl];
+    const float bmin = ((ConstFloat*)a.brick_min)[brick];
     const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
     const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
     const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
     float fmin = __int_as_float(0x7f800000);
-    const int cnt = fuse_column<FMT>(a, ip, col, a.zk + bk * 8, brick * kBrickVoxels + lane, fmin);
+    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane, fmin);
     // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) fmin = fminf(fmin, __shfl_xor(fmin, off, 64));
-    if (lane == 0 && fmin < a.brick_min[brick]) {
+    if (lane == 0 && fmin < bmin) {
         a.brick_min[brick] = fmin;
         const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
         atomicMin(&a.super_key[sidx], order_key(fmin));
@@ -243,8 +270,8 @@ __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, in
     return cnt;
 }
 
-// Sums the lanes' counts of a workgroup into its own slot (no atomics: launches
-// on one stream never run concurrently, and slot b belongs to workgroup b).
+// Sums the lanes' counts of a workgroup into its own slot (slot b belongs to
+// workgroup b; the add returns nothing, so the wave does not wait for it).
 __device__ __forceinline__ void add_touched(unsigned long long* touched, int cnt, int* wave_cnt)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -255,21 +282,30 @@ __device__ __forceinline__ void add_touched(unsigned long long* touched, int cnt
     if (threadIdx.x == 0) {
         int c = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) c += wave_cnt[w];
-        if (c) touched[blockIdx.x] += (unsigned long long)c;
+        if (c) atomicAdd(&touched[blockIdx.x], (unsigned long long)c);
     }
 }
 
-// Pass 2: waves stride over the visible bricks.
+// Pass 2: waves stride over the visible bricks.  The skip flag, the pose, the
+// list length and the wave's first list entry are uniform loads issued
+// together (the list buffer holds an entry per brick of the volume, so reading
+// an entry past the length stays inside it); the next entry is fetched while
+// a brick is fused.
 
 template <int FMT>
-__global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, const int* list,
+__global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs a, const int* list,
                                                                     const unsigned int* count)
 {
-    if (a.dyn && a.dyn->frame_skip_fuse) return;
     __shared__ int wave_cnt[kFuseWaves];
-    // the pose is uniform: scalar loads (constant address space: no kernel
-    // writes the PoseState while this one runs)
+    // constant address space: no kernel writes the PoseState, the list or its
+    // length while this one runs
     typedef __attribute__((address_space(4))) const PoseState ConstState;
+    const int lane = threadIdx.x & 63;
+    const unsigned int wave = (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned int cap = (unsigned)a.nb * (unsigned)a.nb * (unsigned)a.nb;
+    const unsigned int waves = gridDim.x * kFuseWaves;
+    unsigned int item = blockIdx.x * kFuseWaves + wave;
+    const int skip = a.dyn ? ((ConstState*)a.dyn)->frame_skip_fuse : 0;
     float ip[16];
     if (a.dyn) {
         ConstState* cs = (ConstState*)a.dyn;
@@ -279,17 +315,24 @@ __global__ __launch_bounds__(64 * kFuseWaves) void fuse_list_kernel(FuseArgs a, 
 #pragma unroll
         for (int q = 0; q < 16; ++q) ip[q] = a.inv_pose[q];
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned int total = *count;
-    const unsigned int waves = gridDim.x * kFuseWaves;
+    const unsigned int total = *(ConstUint*)count;
+    int brick = item < cap ? ((ConstInt*)list)[item] : 0;
+    if (skip) return;
     int cnt = 0;
-    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves)
-        cnt += fuse_brick<FMT>(a, ip, list[item], lane);
+    while (item < total) {
+        const unsigned int next = item + waves;
+        const int next_brick = next < cap ? ((ConstInt*)list)[next] : 0;
+        cnt += fuse_brick<FMT>(a, ip, brick, lane);
+        item = next;
+        brick = next_brick;
+    }
     add_touched(a.touched, cnt, wave_cnt);
 }
 
 // ---- many sessions per launch (batch ticks).  Jobs are sorted by brick0;
-// global brick g of the tick belongs to the job whose range holds it.
+// global brick g of the tick belongs to the job whose range holds it.  The
+// cull writes list entries job << 32 | brick (the brick's index in the job's
+// volume), so the sweep needs no search.
 constexpr int kBatchCullThreads = 256;
 
 __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(const BatchHeader* hd,
@@ -304,12 +347,15 @@ __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(cons
     for (unsigned base = blockIdx.x * kBatchCullThreads; base < total; base += stride) {
         const unsigned g = base + threadIdx.x;
         bool vis = false;
+        unsigned long long entry = 0;
         if (g < total) {
-            const FuseJob& J = jobs[job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; })];
+            const int j = job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; });
+            const FuseJob& J = jobs[j];
             const FuseArgs& a = J.a;
             if (!a.dyn->frame_skip_fuse) {
                 const float* ip = a.dyn->inv_pose;
                 const int64_t brick = g - J.brick0;
+                entry = (unsigned long long)j << 32 | (unsigned long long)brick;
                 const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
                 const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
                 vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
@@ -330,7 +376,7 @@ __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(cons
         __syncthreads();
         if (vis)
             hd->fuse_list[wave_base[kBatchCullThreads / 64] + wave_base[wave] +
-                          (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = g;
+                          (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = entry;
         __syncthreads();
     }
 }
@@ -360,39 +406,50 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
     a.format = j.a.format;
 }
 
+// Each wave sweeps a contiguous run of the list: consecutive entries mostly
+// belong to one session (the cull appends runs of up to 256 bricks of a job),
+// so the job's arguments and pose are reloaded only when the job changes, and
+// the next entry is fetched while a brick is fused.
 __global__ __launch_bounds__(64 * kFuseWaves) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
 {
     __shared__ int wave_cnt[kFuseWaves];
-    // the tick's table is never written while the kernel runs: read it with
-    // scalar loads through the constant address space
+    // the tick's table, the list and its length are never written while the
+    // kernel runs: read them with scalar loads through the constant address space
     typedef __attribute__((address_space(4))) const FuseJob ConstJob;
     typedef __attribute__((address_space(4))) const BatchHeader ConstHeader;
+    typedef __attribute__((address_space(4))) const PoseState ConstState;
+    typedef __attribute__((address_space(4))) const unsigned long long ConstEntry;
     ConstJob* cj = (ConstJob*)jobs;
     ConstHeader* ch = (ConstHeader*)hd;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned int total = *ch->fuse_count;
+    const int lane = threadIdx.x & 63;
+    const unsigned int wave = (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned int total = *(ConstUint*)ch->fuse_count;
+    ConstEntry* list = (ConstEntry*)ch->fuse_list;
     const unsigned int waves = gridDim.x * kFuseWaves;
-    const int njobs = ch->n_fuse;
-    const unsigned* list = ch->fuse_list;
+    const unsigned int per = (total + waves - 1) / waves;
+    const unsigned long long first = (unsigned long long)(blockIdx.x * kFuseWaves + wave) * per;
+    unsigned int item = first < total ? (unsigned)first : total;
+    const unsigned int end = item + per < total ? item + per : total;
+    unsigned long long e = item < end ? list[item] : 0ull;
+    int cur = -1;
+    FuseArgs a;
+    float ip[16];
     int cnt = 0;
-    for (unsigned int item = blockIdx.x * kFuseWaves + wave; item < total; item += waves) {
-        const unsigned g = __builtin_amdgcn_readfirstlane(list[item]);
-        int lo = 0, hi = njobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cj[mid].brick0 <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        FuseArgs a;
-        load_fuse_args(cj[lo], a);
-        const int64_t brick = g - cj[lo].brick0;
-        typedef __attribute__((address_space(4))) const PoseState ConstState;
-        ConstState* cs = (ConstState*)a.dyn;
-        float ip[16];
+    while (item < end) {
+        const unsigned long long ne = item + 1 < end ? list[item + 1] : 0ull;
+        const int job = (int)(e >> 32);
+        const int64_t brick = (int64_t)(e & 0xffffffffull);
+        if (job != cur) {
+            cur = job;
+            load_fuse_args(cj[job], a);
+            ConstState* cs = (ConstState*)a.dyn;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) ip[q] = cs->inv_pose[q];
+            for (int q = 0; q < 16; ++q) ip[q] = cs->inv_pose[q];
+        }
         if (a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(a, ip, brick, lane);
         else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(a, ip, brick, lane);
+        e = ne;
+        ++item;
     }
     add_touched(hd->touched, cnt, wave_cnt);
 }
@@ -594,6 +651,27 @@ int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_
     return KF_OK;
 }
 
+// Workgroups of the list sweep that are resident at once on the current
+// device (at most kFuseBlocks, the size of the per-workgroup counters).
+static int fuse_list_blocks(int format)
+{
+    static int cached[2] = {0, 0};
+    int& c = cached[format == KF_VOXEL_F32 ? 0 : 1];
+    if (!c) {
+        int dev = 0, cus = 0, per_cu = 0;
+        const void* k = format == KF_VOXEL_F32 ? reinterpret_cast<const void*>(&fuse_list_kernel<KF_VOXEL_F32>)
+                                               : reinterpret_cast<const void*>(&fuse_list_kernel<KF_VOXEL_SNORM16_U8>);
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * kFuseWaves, 0) == hipSuccess && per_cu > 0 &&
+            cus > 0)
+            c = std::min<int>(kFuseBlocks, per_cu * cus);
+        else
+            c = kFuseBlocks;
+    }
+    return c;
+}
+
 float* volume_depth_max(kf_volume* v, int w, int h, hipStream_t s)
 {
     const size_t need = depth_max_floats(w, h);
@@ -626,8 +704,9 @@ int launch_fuse(const float* depth, int w, int h, kf_volume* v, const float* K, 
     fuse_cull_kernel<<<(unsigned)((nbricks + kCullThreads - 1) / kCullThreads), kCullThreads, 0, s>>>(
         a, nbricks, v->brick_list, v->list_count, parity);
     KF_LAUNCHED("fuse_cull_kernel");
-    // 256 CUs x 8 resident workgroups of 4 waves; fewer when the volume is small
-    const unsigned blocks = (unsigned)std::min<int64_t>((nbricks + kFuseWaves - 1) / kFuseWaves, kFuseBlocks);
+    // every workgroup resident at once (no second round); fewer when the volume is small
+    const unsigned blocks =
+        (unsigned)std::min<int64_t>((nbricks + kFuseWaves - 1) / kFuseWaves, fuse_list_blocks(v->format));
     if (list_events) KF_HIP(hipEventRecord(list_events[0], s));
     if (v->format == KF_VOXEL_F32)
         fuse_list_kernel<KF_VOXEL_F32><<<blocks, 64 * kFuseWaves, 0, s>>>(a, v->brick_list, v->list_count + parity);
