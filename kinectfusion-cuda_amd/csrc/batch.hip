@@ -108,7 +108,9 @@ struct kf_batch {
     Slot slot[2];
     int next = 0;
     unsigned* fuse_count = nullptr;
-    unsigned* ray_next = nullptr;  // 16 raycast tile counters
+    unsigned* ray_next = nullptr;  // 16 raycast tile counters + 2 deferred-ray counts
+    kf::DeferredRay* ray_list = nullptr;  // 2 x ray_cap deferred rays (per voxel format)
+    size_t ray_cap = 0;
     unsigned long long* fuse_list = nullptr;
     size_t list_cap = 0;
     unsigned long long* touched = nullptr;
@@ -187,8 +189,34 @@ int drop_graphs(kf_batch* b)
 }
 
 // Total bricks of the resident sessions: the fuse list never holds more.
+// Deferred raycast rays: room for every ray of the resident sessions, per
+// voxel format, up to kMaxDeferred (beyond it a lane marches its ray itself).
+constexpr size_t kMaxDeferred = size_t(1) << 21;
+
+int ensure_ray_list(kf_batch* b)
+{
+    size_t need = 0;
+    for (const Session& q : b->sess)
+        if (q.used) {
+            const size_t w = (size_t)q.cfg.frame_width, h = (size_t)q.cfg.frame_height;
+            need += w * h + (w / 2) * (h / 2) + (w / 4) * (h / 4);
+        }
+    need = std::min(need, kMaxDeferred);
+    if (need <= b->ray_cap) return KF_OK;
+    need = std::min(std::max(need, 2 * b->ray_cap), kMaxDeferred);
+    KF_HIP(hipStreamSynchronize(b->s));
+    (void)hipFree(b->ray_list);
+    b->ray_list = nullptr;
+    b->ray_cap = 0;
+    int st = alloc((void**)&b->ray_list, 2 * sizeof(kf::DeferredRay) * need, nullptr);
+    if (st == KF_OK) b->ray_cap = need;
+    return st;
+}
+
 int ensure_list(kf_batch* b)
 {
+    int rs = ensure_ray_list(b);
+    if (rs) return rs;
     size_t need = 0;
     for (const Session& q : b->sess)
         if (q.used) need += (size_t)(q.cfg.volume_n / 8) * (q.cfg.volume_n / 8) * (q.cfg.volume_n / 8);
@@ -359,6 +387,7 @@ int kf_batch_destroy(kf_batch* b)
         if (q.used) free_session(q);
     (void)hipFree(b->fuse_count);
     (void)hipFree(b->ray_next);
+    (void)hipFree(b->ray_list);
     (void)hipFree(b->fuse_list);
     (void)hipFree(b->touched);
     if (b->own_stream && b->s) (void)hipStreamDestroy(b->s);
@@ -410,7 +439,7 @@ int kf_batch_create(const kf_batch_config* cfg, kf_stream_t stream, kf_batch** o
             if (!st && hipEventCreate(&ev) != hipSuccess) st = kf::fail(KF_ERR_HIP, "kf_batch_create events");
     }
     if (!st) st = alloc((void**)&b->fuse_count, sizeof(unsigned), nullptr);
-    if (!st) st = alloc((void**)&b->ray_next, 16 * sizeof(unsigned), nullptr);
+    if (!st) st = alloc((void**)&b->ray_next, 18 * sizeof(unsigned), nullptr);
     if (!st) st = alloc((void**)&b->touched, sizeof(unsigned long long) * kMaxFuseBlocks, nullptr);
     if (!st) {
         hipError_t e = hipMemset(b->touched, 0, sizeof(unsigned long long) * kMaxFuseBlocks);
@@ -531,6 +560,8 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
     memset(hd, 0, sizeof(*hd));
     hd->fuse_count = b->fuse_count;
     hd->ray_next = b->ray_next;
+    hd->ray_list = b->ray_list;
+    hd->ray_list_cap = (unsigned)b->ray_cap;
     hd->fuse_list = b->fuse_list;
     hd->touched = b->touched;
     const kf_icp_config& ic = b->cfg.icp;

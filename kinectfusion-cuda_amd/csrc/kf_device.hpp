@@ -176,6 +176,19 @@ __device__ __forceinline__ int64_t brick_index(int nb, int i, int j, int k)
     int64_t b = ((int64_t)(k >> 3) * nb + (j >> 3)) * nb + (i >> 3);
     return b * kBrickVoxels + (((k & 7) * 8 + (j & 7)) * 8 + (i & 7));
 }
+// The same index as a sum of per-axis terms, brick_index = ox(i) + oy(j) + oz(k),
+// in 32 bits (volumes are at most 1024^3 voxels): the 8 corners of a trilinear
+// cell then cost 6 terms and 12 additions, and a load is a 32-bit offset from
+// the plane's base.
+__device__ __forceinline__ uint32_t brick_ox(int i) { return (uint32_t)(i >> 3) * 512u + (uint32_t)(i & 7); }
+__device__ __forceinline__ uint32_t brick_oy(int nb, int j)
+{
+    return (uint32_t)(j >> 3) * (uint32_t)nb * 512u + (uint32_t)(j & 7) * 8u;
+}
+__device__ __forceinline__ uint32_t brick_oz(int nb, int k)
+{
+    return (uint32_t)(k >> 3) * (uint32_t)nb * (uint32_t)nb * 512u + (uint32_t)(k & 7) * 64u;
+}
 
 // Super-bricks: 8x8x8 bricks (64^3 voxels) carry a lower bound of F too, kept
 // as an ordered integer key so it can be lowered with an integer atomicMin.

@@ -83,6 +83,12 @@ struct kf_volume {
     size_t dmax_cap = 0;                    // floats allocated for dmax
     int64_t fuse_launches = 0;
     size_t bytes = 0;
+    // raycast scratch (grown on demand, also by raycasts through a const volume):
+    // deferred-ray list and its two launch-parity counters
+    mutable void* ray_list = nullptr;
+    mutable unsigned* ray_count = nullptr;
+    mutable size_t ray_cap = 0;
+    mutable int64_t ray_launches = 0;
 };
 
 struct kf_icp {

@@ -29,7 +29,10 @@ struct BatchHeader {
     unsigned* fuse_count;
     unsigned long long* fuse_list;    // job << 32 | brick of the job's volume
     unsigned long long* touched;
-    unsigned* ray_next;  // 16 tile counters (8 XCD bands x 2 voxel formats), cleared by the first kernel
+    unsigned* ray_next;  // 16 tile counters (8 XCD bands x 2 voxel formats) and 2 deferred-ray
+                         // counts (per format), cleared by the first kernel
+    struct DeferredRay* ray_list;  // 2 x ray_list_cap deferred rays (F32 jobs, then snorm16 jobs)
+    unsigned ray_list_cap;
 };
 
 // Preprocess of one session: raw codes -> metres (fuse input), bilateral L0,
@@ -114,6 +117,15 @@ struct RaycastLevel {
     float4* vout;
     float4* nout;
     int w, h, tiles_x, tile0;  // tile0: first global tile of this level
+};
+// A ray the per-lane march handed to the finishing pass: its march state
+// before the sample at t (that sample already counted in guard).
+struct DeferredRay {
+    int job;    // pyramid level (single launch) or raycast job (batch)
+    int px;     // j * w + i
+    int guard;  // samples counted, the one at t included
+    int flags;  // bit 0: the last sample was skipped (prev comes from prev_t), bit 1: small steps
+    float t, prev, prev_t, pad;
 };
 // One (session, level) of a batched raycast; lives in device memory.
 struct RaycastJob {

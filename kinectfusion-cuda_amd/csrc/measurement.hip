@@ -204,7 +204,8 @@ __global__ __launch_bounds__(256) void measure_batch_kernel(const BatchHeader* h
     __shared__ float wave_max[4];
     if (blockIdx.x == 0 && threadIdx.y == 0) {
         if (threadIdx.x == 0) *hd->fuse_count = 0u;
-        hd->ray_next[threadIdx.x] = 0u;  // kTile == 16 counters
+        hd->ray_next[threadIdx.x] = 0u;  // kTile == 16 tile counters
+        if (threadIdx.x < 2) hd->ray_next[16 + threadIdx.x] = 0u;  // deferred-ray counts
     }
     const int tiles = hd->pre_tiles, njobs = hd->n_pre;
     for (int tl = blockIdx.x; tl < tiles; tl += gridDim.x) {

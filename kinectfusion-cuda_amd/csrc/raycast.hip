@@ -42,6 +42,10 @@ struct RaycastArgs {
     const PoseState* dyn;  // when set, the pose comes from dyn->pose
     RaycastLevel lv[3];
     int levels, tiles;
+    DeferredRay* rays;  // rays handed to the finishing pass (at most cap)
+    unsigned* count;    // their count for this launch
+    unsigned* clear;    // the count of the other launch parity, cleared for the next launch
+    unsigned cap;
 };
 
 template <int FMT>
@@ -79,13 +83,42 @@ __device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
     return st;
 }
 
+// Voxel q of the f plane (32-bit element offset from the plane's base).
+template <int FMT>
+__device__ __forceinline__ float voxel_at(const void* f, uint32_t q)
+{
+    const char* base = static_cast<const char*>(f);
+    if (FMT == KF_VOXEL_F32) return *reinterpret_cast<const float*>(base + (size_t)(q * 4u));
+    return snorm_decode(*reinterpret_cast<const int16_t*>(base + (size_t)(q * 2u)));
+}
+
+// Element offsets of the 8 corners of a stencil's cell, in the order
+// c000 c100 c010 c110 c001 c101 c011 c111 (i fastest).
+__device__ __forceinline__ void cell_offsets(int nb, const Stencil& t, uint32_t* o)
+{
+    const uint32_t x0 = brick_ox(t.i0), x1 = brick_ox(t.i1);
+    const uint32_t y0 = brick_oy(nb, t.j0), y1 = brick_oy(nb, t.j1);
+    const uint32_t z0 = brick_oz(nb, t.k0), z1 = brick_oz(nb, t.k1);
+    const uint32_t y0z0 = y0 + z0, y1z0 = y1 + z0, y0z1 = y0 + z1, y1z1 = y1 + z1;
+    o[0] = x0 + y0z0;
+    o[1] = x1 + y0z0;
+    o[2] = x0 + y1z0;
+    o[3] = x1 + y1z0;
+    o[4] = x0 + y0z1;
+    o[5] = x1 + y0z1;
+    o[6] = x0 + y1z1;
+    o[7] = x1 + y1z1;
+}
+
 template <int FMT>
 __device__ __forceinline__ float trilinear(const void* F, int nb, const Stencil& t)
 {
-    float c000 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k0)), c100 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k0));
-    float c010 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k0)), c110 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k0));
-    float c001 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k1)), c101 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k1));
-    float c011 = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k1)), c111 = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k1));
+    uint32_t o[8];
+    cell_offsets(nb, t, o);
+    float c000 = voxel_at<FMT>(F, o[0]), c100 = voxel_at<FMT>(F, o[1]);
+    float c010 = voxel_at<FMT>(F, o[2]), c110 = voxel_at<FMT>(F, o[3]);
+    float c001 = voxel_at<FMT>(F, o[4]), c101 = voxel_at<FMT>(F, o[5]);
+    float c011 = voxel_at<FMT>(F, o[6]), c111 = voxel_at<FMT>(F, o[7]);
     const float a = t.a, b = t.b, c = t.c;
     float e00 = (1.0f - a) * c000 + a * c100, e10 = (1.0f - a) * c010 + a * c110;
     float e01 = (1.0f - a) * c001 + a * c101, e11 = (1.0f - a) * c011 + a * c111;
@@ -161,14 +194,10 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
         cc.ki = ki;
         cc.kj = kj;
         cc.kk = kk;
-        cc.c[0] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k0));
-        cc.c[1] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k0));
-        cc.c[2] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k0));
-        cc.c[3] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k0));
-        cc.c[4] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j0, t.k1));
-        cc.c[5] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j0, t.k1));
-        cc.c[6] = voxel_f<FMT>(F, brick_index(nb, t.i0, t.j1, t.k1));
-        cc.c[7] = voxel_f<FMT>(F, brick_index(nb, t.i1, t.j1, t.k1));
+        uint32_t o[8];
+        cell_offsets(nb, t, o);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cc.c[q] = voxel_at<FMT>(F, o[q]);
     }
     const float a = t.a, b = t.b, c = t.c;
     float e00 = (1.0f - a) * cc.c[0] + a * cc.c[1], e10 = (1.0f - a) * cc.c[2] + a * cc.c[3];
@@ -177,12 +206,14 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
     return (1.0f - c) * g0 + c * g1;
 }
 
-// A ray that has taken kSmallBudget small steps is handed to the whole wave
-// once no more than kCoopLanes lanes of the wave are still marching (about 1
-// ray in 40 grazes a surface and takes hundreds of small steps; while many
-// lanes march, the wave is busy anyway).
-constexpr int kSmallBudget = 32;
-constexpr int kCoopLanes = 16;
+// A lane marches at most kLaneBudget samples of its ray unless at most
+// kWorkFloor of them needed voxel reads (a ray crossing free space to a distant
+// surface is cheap and stays); a longer ray (about 1 in 40 grazes a surface and
+// takes hundreds of small steps) is appended to a list that a second pass
+// finishes with a whole wave per ray, so no wave of the first pass waits on
+// one long ray.
+constexpr int kLaneBudget = 48;
+constexpr int kWorkFloor = 16;
 
 // One trilinear evaluation of the march at t: 1 = skipped (the sample is known
 // to be >= 0.99), 0 = evaluated into f.
@@ -202,36 +233,62 @@ __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir
     return 0;
 }
 
+// Small steps are taken kGroup at a time: the group's positions come from the
+// same repeated additions, its skip tests use only the bounds a lane has
+// already cached, and the corner loads of all its evaluated samples are issued
+// together, so a group costs one memory round trip instead of kGroup.  The
+// samples are then consumed in order exactly as single steps would be (a
+// sample past an event is discarded).
+constexpr int kGroup = 2;
+
+// fast_skip without memory access: 1 when the cached super-brick or brick bound
+// shows the exact stencil is >= 0.99, else 0 (evaluate the sample).
+__device__ __forceinline__ int cached_skip(const RaycastVolume& a, float xa, float ya, float za, int cs, float cb,
+                                           int ck, float cbk)
+{
+    const float hi = (float)a.n - 1.0f - kMargin;
+    if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return 0;
+    const int xi = (int)xa, yi = (int)ya, zi = (int)za;
+    const int sx = xi >> 6, sy = yi >> 6, sz = zi >> 6;
+    const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
+    const float shi = (float)(kSuper - 1) - kMargin;
+    if (ux >= kMargin && uy >= kMargin && uz >= kMargin && ux < shi && uy < shi && uz < shi &&
+        (sz * a.ns + sy) * a.ns + sx == cs && cb >= kSkipBound)
+        return 1;
+    const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
+    const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
+    const float bhi = (float)(kBrick - 1) - kMargin;
+    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi &&
+        (bz * a.nb + by) * a.nb + bx == ck && cbk >= kSkipBound)
+        return 1;
+    return 0;
+}
+
 __device__ __forceinline__ float lane_bcast(float v, int lane)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// raycast.cu:44-145 for pixel (i, j) of one level map.  Called by all 64
-// lanes of a wave together (in_image false for lanes past the map's edge).
-// Phase 1: each lane marches its own ray; a long ray is deferred once few
-// lanes are left marching (kSmallBudget, kCoopLanes).  Phase 2: the wave marches each
-// deferred ray 64 samples at a time, lane k taking sample k of the chunk (its
-// position by the same repeated additions t = t + step), and a ballot finds
-// the first negative sample.  Every sample position, value and decision is the
-// sequential march's, so the output is the same.
-template <int FMT>
-__device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i,
-                                              int j, bool in_image)
+// Camera ray of pixel (i, j) and its march interval (raycast.cu:7-27, 44-73).
+struct RayGeom {
+    v3 o, dir;
+    float nearv, farv;
+    bool active;  // false: the pixel is left untouched (raycast.cu:61-73)
+};
+__device__ __forceinline__ RayGeom ray_geom(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i,
+                                            int j, bool in_image)
 {
-    const int lane = threadIdx.x & 63;
     float Rm[9] = {pose[0], pose[1], pose[2], pose[4], pose[5], pose[6], pose[8], pose[9], pose[10]};
     float M[9];
     m3m3(Rm, L.Kinv, M);
-    const v3 o = mk3(pose[12], pose[13], pose[14]);
-    const float width = a.width, mue = a.mue, hw = width * 0.5f;
-
-    v3 dir = m3v(M, mk3((float)i + 0.5f, (float)j + 0.5f, 1.0f));
-    v3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-    // raycast.cu:7-27
-    float t0x = (-hw - o.x) * inv.x, t1x = (hw - o.x) * inv.x;
-    float t0y = (-hw - o.y) * inv.y, t1y = (hw - o.y) * inv.y;
-    float t0z = (-hw - o.z) * inv.z, t1z = (hw - o.z) * inv.z;
+    RayGeom g;
+    g.o = mk3(pose[12], pose[13], pose[14]);
+    const float hw = a.width * 0.5f;
+    g.dir = m3v(M, mk3((float)i + 0.5f, (float)j + 0.5f, 1.0f));
+    const v3 inv = mk3(1.0f / g.dir.x, 1.0f / g.dir.y, 1.0f / g.dir.z);
+    float t0x = (-hw - g.o.x) * inv.x, t1x = (hw - g.o.x) * inv.x;
+    float t0y = (-hw - g.o.y) * inv.y, t1y = (hw - g.o.y) * inv.y;
+    float t0z = (-hw - g.o.z) * inv.z, t1z = (hw - g.o.z) * inv.z;
     float tmin = fminf(t0x, t1x), tmax = fmaxf(t0x, t1x);
     tmin = fmaxf(tmin, fminf(t0y, t1y));
     tmax = fminf(tmax, fmaxf(t0y, t1y));
@@ -241,128 +298,26 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         tmin = -1.0f;
         tmax = -1.0f;
     }
-    const float nearv = (kMinDistance < tmin) ? tmin : kMinDistance;
-    const float farv = (tmax < kMaxDistance) ? tmax : kMaxDistance;
-    // raycast.cu:61-73: a ray behind the camera or with near >= far leaves the pixel untouched
-    const bool active = in_image && !(tmax < 0.0f) && nearv < farv;
+    g.nearv = (kMinDistance < tmin) ? tmin : kMinDistance;
+    g.farv = (tmax < kMaxDistance) ? tmax : kMaxDistance;
+    g.active = in_image && !(tmax < 0.0f) && g.nearv < g.farv;
+    return g;
+}
 
-    // raycast.cu:75-117.  A skipped sample only sets prev; its value is
-    // computed if and when the zero-crossing formula needs it.
-    const float large = mue * 1.99f, small = mue * 0.1f;
-    const float invw = 1.0f / width, nf = (float)a.n;
-    float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f, t = nearv;
-    bool prev_skipped = false, deferred = false;
-    int cs = -1, ck = -1;
-    float cb = 0.0f, cbk = 0.0f;
-    CellCache cc;
-    int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
-    if (active) {
-        int smalls = 0;
-        for (; t < farv; t = t + step) {
-            if (++guard > kMaxMarchSteps) break;
-            const int marching = __popcll(__ballot(1));  // lanes of the wave still in this loop
-            if (step == small && ++smalls > kSmallBudget && marching <= kCoopLanes) {
-                deferred = true;  // this sample (guard already counted) is the wave's first
-                break;
-            }
-            float f = 0.0f;
-            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, cs, cb, ck, cbk, cc, f)) {
-                prev_skipped = true;
-                prev_t = t;
-                continue;
-            }
-            if (f < 0.0f) {
-                if (t != nearv) {
-                    if (step == large) {
-                        t = t - mue;
-                        step = small;
-                        continue;
-                    }
-                    if (prev_skipped) {
-                        v3 pp = add3(o, mul3s(dir, prev_t));
-                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
-                                                pp.z / width + 0.5f);
-                    }
-                    tstar = t - step * prev / (f - prev);
-                }
-                break;
-            } else if (f < 0.99f) {
-                step = small;
-            }
-            prev = f;
-            prev_skipped = false;
-        }
-    }
-
-    // phase 2: the deferred rays of the wave, one at a time, 64 samples per chunk
-    uint64_t pending = __ballot(deferred);
-    while (pending) {
-        const int r = __ffsll((unsigned long long)pending) - 1;
-        pending &= pending - 1;
-        const v3 dr = mk3(lane_bcast(dir.x, r), lane_bcast(dir.y, r), lane_bcast(dir.z, r));
-        const float fr = lane_bcast(farv, r);
-        float tb = lane_bcast(t, r);  // position of sample 0 of the chunk
-        float pv = lane_bcast(prev, r), pt = lane_bcast(prev_t, r);
-        int psk = __builtin_amdgcn_readlane(prev_skipped ? 1 : 0, r);
-        int gb = __builtin_amdgcn_readlane(guard, r);  // guard count of sample 0
-        float ts = 0.0f;
-        for (;;) {
-            float tk = tb;
-            for (int q = 0; q < lane; ++q) tk = tk + small;
-            // sample k exists if every sample up to it passes the loop test and the guard
-            const bool end = !(tk < fr) || gb + lane > kMaxMarchSteps;
-            float f = 0.0f;
-            int sk = 1;
-            if (!end) sk = march_sample<FMT>(a, o, dr, tk, width, invw, nf, cs, cb, ck, cbk, cc, f);
-            const bool neg = !end && !sk && f < 0.0f;
-            const uint64_t ev = __ballot(end || neg);
-            if (ev) {
-                const int k = __ffsll((unsigned long long)ev) - 1;
-                const bool kneg = __builtin_amdgcn_readlane(neg ? 1 : 0, k) != 0;
-                if (kneg) {
-                    // prev = the value of sample k - 1 (exactly, also if it was skipped)
-                    float pk;
-                    if (k > 0) {
-                        float fe = f;
-                        if (lane == k - 1 && sk) {
-                            v3 pp = add3(o, mul3s(dr, tk));
-                            fe = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
-                                                  pp.z / width + 0.5f);
-                        }
-                        pk = lane_bcast(fe, k - 1);
-                    } else if (psk) {
-                        v3 pp = add3(o, mul3s(dr, pt));
-                        pk = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
-                                              pp.z / width + 0.5f);
-                    } else {
-                        pk = pv;
-                    }
-                    const float tsk = tk - small * pk / (f - pk);
-                    ts = lane_bcast(tsk, k);
-                }
-                break;
-            }
-            // no event: the chunk's last sample becomes prev, the next chunk follows it
-            const float t63 = lane_bcast(tk, 63);
-            psk = __builtin_amdgcn_readlane(sk, 63);
-            pv = lane_bcast(f, 63);
-            pt = t63;
-            tb = t63 + small;
-            gb += 64;
-        }
-        if (lane == r) tstar = ts;
-    }
-
-    if (!active) return;
+// raycast.cu:29-42 and 119-145: the surface point at tstar (tstar == 0: no
+// crossing) and its normal -grad F, written by one lane.  The gradient takes
+// its 6 samples one axis at a time (fewer live registers); x + 0 == x, so the
+// samples are those of raycast.cu:33-38.
+template <int FMT>
+__device__ __forceinline__ void write_surface(const RaycastVolume& a, const RaycastLevel& L, int i, int j,
+                                              const RayGeom& g, float tstar)
+{
     const int64_t q = (int64_t)j * L.w + i;
     if (tstar != 0.0f) {
-        v3 vtx = add3(o, mul3s(dir, tstar));
-        // raycast.cu:29-42
-        float r = mue * 0.5f / width;
-        float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
-        // one axis at a time (16 loads in flight, not 48): fewer live registers,
-        // more waves per SIMD for the march.  x + 0 == x, so the samples are the
-        // six of raycast.cu:33-38.
+        const float width = a.width;
+        const v3 vtx = add3(g.o, mul3s(g.dir, tstar));
+        const float r = a.mue * 0.5f / width;
+        const float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll 1
         for (int ax = 0; ax < 3; ++ax) {
@@ -374,10 +329,10 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
             gy = ax == 1 ? d : gy;
             gz = ax == 2 ? d : gz;
         }
-        v3 g = div3s(mk3(gx, gy, gz), r);
-        float gn = len3(g);
+        const v3 gg = div3s(mk3(gx, gy, gz), r);
+        const float gn = len3(gg);
         if (is_depth_valid(gn)) {
-            v3 nn = div3s(g, gn);
+            const v3 nn = div3s(gg, gn);
             L.vout[q] = make_float4(vtx.x, vtx.y, vtx.z, kValid);
             L.nout[q] = make_float4(nn.x, nn.y, nn.z, kValid);
             return;
@@ -385,6 +340,344 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     }
     L.vout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
     L.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+}
+
+// The same, with the 6 gradient samples taken by lanes 0-5 of the wave and the
+// result written by lane 0 (the finishing pass: one ray per wave).
+template <int FMT>
+__device__ __forceinline__ void write_surface_wave(const RaycastVolume& a, const RaycastLevel& L, int i, int j,
+                                                   const RayGeom& g, float tstar)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)j * L.w + i;
+    if (tstar != 0.0f) {
+        const float width = a.width;
+        const v3 vtx = add3(g.o, mul3s(g.dir, tstar));
+        const float r = a.mue * 0.5f / width;
+        const float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
+        float val = 0.0f;
+        if (lane < 6) {
+            const int ax = lane >> 1;
+            const float du = ax == 0 ? r : 0.0f, dv = ax == 1 ? r : 0.0f, ds = ax == 2 ? r : 0.0f;
+            if (lane & 1) val = sample_tsdf<FMT>(a.f, a.n, a.nb, u - du, v - dv, s - ds);
+            else val = sample_tsdf<FMT>(a.f, a.n, a.nb, u + du, v + dv, s + ds);
+        }
+        const float gx = lane_bcast(val, 1) - lane_bcast(val, 0);
+        const float gy = lane_bcast(val, 3) - lane_bcast(val, 2);
+        const float gz = lane_bcast(val, 5) - lane_bcast(val, 4);
+        const v3 gg = div3s(mk3(gx, gy, gz), r);
+        const float gn = len3(gg);
+        if (is_depth_valid(gn)) {
+            if (lane == 0) {
+                const v3 nn = div3s(gg, gn);
+                L.vout[q] = make_float4(vtx.x, vtx.y, vtx.z, kValid);
+                L.nout[q] = make_float4(nn.x, nn.y, nn.z, kValid);
+            }
+            return;
+        }
+    }
+    if (lane == 0) {
+        L.vout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+        L.nout[q] = make_float4(0.0f, 0.0f, 0.0f, kInvalid);
+    }
+}
+
+// Where the first pass appends the rays it hands over.
+struct RayQueue {
+    DeferredRay* rays;
+    unsigned* count;
+    unsigned cap;
+};
+
+// raycast.cu:44-145 for pixel (i, j) of one level map, one lane per pixel,
+// called by all 64 lanes of a wave together (in_image false past the map's
+// edge).  A ray still marching after its budget of samples is appended to q with
+// its exact march state and left to the finishing pass (when q is full it
+// marches on here).  A skipped sample only sets prev_t; its value is computed
+// if and when the zero-crossing formula needs it.
+template <int FMT>
+__device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i,
+                                              int j, bool in_image, const RayQueue& rq, int job)
+{
+    const RayGeom g = ray_geom(a, pose, L, i, j, in_image);
+    const v3 o = g.o, dir = g.dir;
+    const float nearv = g.nearv, farv = g.farv;
+    const float width = a.width, mue = a.mue;
+    const float large = mue * 1.99f, small = mue * 0.1f;
+    const float invw = 1.0f / width, nf = (float)a.n;
+    float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f, t = nearv;
+    bool prev_skipped = false, done = false, can_defer = true;
+    int work = 0;  // samples evaluated (memory read) so far
+    int cs = -1, ck = -1;
+    float cb = 0.0f, cbk = 0.0f;
+    CellCache cc;
+    int guard = 0;  // never binds for mue >= 1e-4 m (kf_volume_create enforces it)
+    if (!g.active) return;
+    for (;;) {
+        if (!(t < farv)) break;
+        if (++guard > kMaxMarchSteps) break;
+        if (can_defer && guard > kLaneBudget && work > kWorkFloor) {
+            const unsigned slot = atomicAdd(rq.count, 1u);
+            if (slot < rq.cap) {
+                DeferredRay d;
+                d.job = job;
+                d.px = j * L.w + i;
+                d.guard = guard;
+                d.flags = (prev_skipped ? 1 : 0) | (step == small ? 2 : 0);
+                d.t = t;
+                d.prev = prev;
+                d.prev_t = prev_t;
+                d.pad = 0.0f;
+                rq.rays[slot] = d;
+                return;
+            }
+            can_defer = false;
+        }
+        if (step == large) {
+            float f = 0.0f;
+            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, cs, cb, ck, cbk, cc, f)) {
+                prev_skipped = true;
+                prev_t = t;
+                t = t + step;
+                continue;
+            }
+            ++work;
+            if (f < 0.0f) {
+                if (t == nearv) break;
+                t = t - mue;  // back off and march on with small steps
+                step = small;
+                t = t + step;
+                continue;
+            }
+            if (f < 0.99f) step = small;
+            prev = f;
+            prev_skipped = false;
+            t = t + step;
+            continue;
+        }
+        // a group of small samples: sample 0 at t is counted; sample k exists while
+        // the loop test and the guard would let the single-step march reach it
+        float tq[kGroup];
+        bool ex[kGroup], ev[kGroup];
+        Stencil st[kGroup];
+        tq[0] = t;
+        ex[0] = true;
+#pragma unroll
+        for (int k = 1; k < kGroup; ++k) {
+            tq[k] = tq[k - 1] + small;
+            ex[k] = ex[k - 1] && tq[k] < farv && guard + k <= kMaxMarchSteps;
+        }
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) {
+            const v3 p = add3(o, mul3s(dir, tq[k]));
+            const int sk = a.brick_min ? cached_skip(a, (p.x * invw + 0.5f) * nf - 0.5f,
+                                                     (p.y * invw + 0.5f) * nf - 0.5f,
+                                                     (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
+                                       : 0;
+            st[k] = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+            bool skip = sk == 1;
+            if (!skip && a.brick_min) {
+                const Stencil& q = st[k];
+                const int b0 = ((q.k0 >> 3) * a.nb + (q.j0 >> 3)) * a.nb + (q.i0 >> 3);
+                const int b1 = ((q.k1 >> 3) * a.nb + (q.j1 >> 3)) * a.nb + (q.i1 >> 3);
+                skip = b0 == b1 && b0 == ck && cbk >= kSkipBound;
+            }
+            ev[k] = ex[k] && !skip;
+        }
+        // the bounds of the last existing sample's brick and super-brick (for the
+        // next group's tests; selects, no dynamic index), then the corner loads of
+        // every evaluated sample
+        if (a.brick_min) {
+            int bidx = 0, sidx = 0;
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                const Stencil& q = st[k];
+                const int bk = ((q.k0 >> 3) * a.nb + (q.j0 >> 3)) * a.nb + (q.i0 >> 3);
+                const int sk = ((q.k0 >> 6) * a.ns + (q.j0 >> 6)) * a.ns + (q.i0 >> 6);
+                bidx = ex[k] ? bk : bidx;
+                sidx = ex[k] ? sk : sidx;
+            }
+            if (bidx != ck) {
+                ck = bidx;
+                cbk = a.brick_min[bidx];
+            }
+            if (sidx != cs) {
+                cs = sidx;
+                cb = order_value(a.super_key[sidx]);
+            }
+        }
+        // a sample whose cell is the cached cell (or an earlier sample's of this
+        // group) takes those corners; the others' loads are issued first, the
+        // copies made after them, so the group still costs one round trip
+        float cv[kGroup][8];
+        int src[kGroup];  // -2: the cached cell, -1: own loads, j: sample j of the group
+        {
+            int ki = cc.ki, kj = cc.kj, kk = cc.kk, from = -2;
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                src[k] = -1;
+                if (!ev[k]) continue;
+                const int qi = st[k].i0 | (st[k].i1 << 16), qj = st[k].j0 | (st[k].j1 << 16),
+                          qk = st[k].k0 | (st[k].k1 << 16);
+                if (qi == ki && qj == kj && qk == kk) {
+                    src[k] = from;
+                    continue;
+                }
+                uint32_t o8[8];
+                cell_offsets(a.nb, st[k], o8);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cv[k][q] = voxel_at<FMT>(a.f, o8[q]);
+                ki = qi;
+                kj = qj;
+                kk = qk;
+                from = k;
+            }
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    float v = src[k] == -2 ? cc.c[q] : cv[k][q];
+#pragma unroll
+                    for (int jj = 0; jj < k; ++jj) v = src[k] == jj ? cv[jj][q] : v;
+                    cv[k][q] = v;
+                }
+            }
+            // the cache keeps the last evaluated sample's cell
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k) {
+                if (!ev[k]) continue;
+                cc.ki = st[k].i0 | (st[k].i1 << 16);
+                cc.kj = st[k].j0 | (st[k].j1 << 16);
+                cc.kk = st[k].k0 | (st[k].k1 << 16);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cc.c[q] = cv[k][q];
+            }
+        }
+        // consume the samples in order
+        bool stop = false;
+        t = tq[kGroup - 1] + small;  // where the march goes on if no event intervenes
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) {
+            if (stop) continue;
+            if (!ex[k]) {
+                t = tq[k];  // the loop test or the guard ends the march here
+                if (tq[k] < farv) guard = kMaxMarchSteps + 1;
+                stop = true;
+                continue;
+            }
+            if (k > 0) ++guard;
+            if (!ev[k]) {
+                prev_skipped = true;
+                prev_t = tq[k];
+                continue;
+            }
+            ++work;
+            const float ca = st[k].a, cbw = st[k].b, cc2 = st[k].c;
+            const float e00 = (1.0f - ca) * cv[k][0] + ca * cv[k][1], e10 = (1.0f - ca) * cv[k][2] + ca * cv[k][3];
+            const float e01 = (1.0f - ca) * cv[k][4] + ca * cv[k][5], e11 = (1.0f - ca) * cv[k][6] + ca * cv[k][7];
+            const float g0 = (1.0f - cbw) * e00 + cbw * e10, g1 = (1.0f - cbw) * e01 + cbw * e11;
+            const float f = (1.0f - cc2) * g0 + cc2 * g1;
+            if (f < 0.0f) {
+                if (tq[k] != nearv) {
+                    if (prev_skipped) {
+                        const v3 pp = add3(o, mul3s(dir, prev_t));
+                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
+                                                pp.z / width + 0.5f);
+                    }
+                    tstar = tq[k] - small * prev / (f - prev);
+                }
+                t = tq[k];
+                stop = true;
+                done = true;
+                continue;
+            }
+            prev = f;
+            prev_skipped = false;
+        }
+        if (done || (stop && !(t < farv)) || guard > kMaxMarchSteps) break;
+    }
+    write_surface<FMT>(a, L, i, j, g, tstar);
+}
+
+// The finishing pass for one handed-over ray: the wave marches it 64 samples
+// at a time, lane k taking sample k of the chunk (its position by the same
+// repeated additions t = t + step), and a ballot finds the first sample that
+// ends the ray or changes the march (end of the interval or the guard, f < 0,
+// or f < 0.99 on a large step).  The samples before it are consumed as the
+// single-lane march would (prev from the last of them), the ones after it are
+// discarded, so every sample position, value and decision is the sequential
+// march's and the output is the same.
+template <int FMT>
+__device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* pose, const RaycastLevel& L,
+                                           const DeferredRay& r)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = r.px % L.w, j = r.px / L.w;
+    const RayGeom g = ray_geom(a, pose, L, i, j, true);
+    const float width = a.width, mue = a.mue;
+    const float large = mue * 1.99f, small = mue * 0.1f;
+    const float invw = 1.0f / width, nf = (float)a.n;
+    float step = (r.flags & 2) ? small : large;
+    float t = r.t, pv = r.prev, pt = r.prev_t;
+    bool psk = (r.flags & 1) != 0;
+    int gb = r.guard;
+    int cs = -1, ck = -1;
+    float cb = 0.0f, cbk = 0.0f;
+    CellCache cc;
+    float tstar = 0.0f;
+    for (;;) {
+        float tk = t;
+        for (int q = 0; q < lane; ++q) tk = tk + step;
+        const bool end = !(tk < g.farv) || gb + lane > kMaxMarchSteps;
+        float f = 0.0f;
+        int sk = 1;
+        if (!end) sk = march_sample<FMT>(a, g.o, g.dir, tk, width, invw, nf, cs, cb, ck, cbk, cc, f);
+        const bool neg = !end && !sk && f < 0.0f;
+        const bool slow = !end && !sk && step == large && f < 0.99f;
+        const uint64_t evm = __ballot(end || neg || slow);
+        const int k = evm ? __ffsll((unsigned long long)evm) - 1 : 64;
+        if (k > 0) {
+            // sample k - 1 is the last one consumed: it sets prev
+            if (__builtin_amdgcn_readlane(sk, k - 1)) {
+                psk = true;
+                pt = lane_bcast(tk, k - 1);
+            } else {
+                psk = false;
+                pv = lane_bcast(f, k - 1);
+            }
+        }
+        if (k == 64) {
+            t = lane_bcast(tk, 63) + step;
+            gb += 64;
+            continue;
+        }
+        const float tkk = lane_bcast(tk, k), fk = lane_bcast(f, k);
+        if (__builtin_amdgcn_readlane(end ? 1 : 0, k)) break;
+        if (__builtin_amdgcn_readlane(neg ? 1 : 0, k)) {
+            if (tkk == g.nearv) break;
+            if (step == large) {
+                t = tkk - mue;  // back off and march on with small steps
+                step = small;
+                t = t + step;
+                gb = gb + k + 1;
+                continue;
+            }
+            float pk = pv;
+            if (psk) {
+                const v3 pp = add3(g.o, mul3s(g.dir, pt));
+                pk = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f, pp.z / width + 0.5f);
+            }
+            tstar = tkk - step * pk / (fk - pk);
+            break;
+        }
+        // a large step found f < 0.99: small steps from the next sample on
+        step = small;
+        pv = fk;
+        psk = false;
+        t = tkk + step;
+        gb = gb + k + 1;
+    }
+    write_surface_wave<FMT>(a, L, i, j, g, tstar);
 }
 
 // XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
@@ -397,8 +690,9 @@ __device__ __forceinline__ int xcd_tile(int tiles)
 }
 
 template <int FMT>
-__global__ __launch_bounds__(64, 7) void raycast_kernel(RaycastArgs a)
+__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.clear = 0u;
     const int tile = xcd_tile(a.tiles);
     if (tile >= a.tiles) return;
     int l = 0;
@@ -407,7 +701,34 @@ __global__ __launch_bounds__(64, 7) void raycast_kernel(RaycastArgs a)
     const int lt = tile - L.tile0;
     const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
     const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h);
+    const RayQueue rq{a.rays, a.count, a.cap};
+    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h, rq, l);
+}
+
+// Workgroups (one wave each) of the finishing pass; each takes every
+// kFinishWaves-th handed-over ray.
+constexpr unsigned kFinishWaves = 4096;
+
+template <int FMT>
+__global__ __launch_bounds__(64) void raycast_finish_kernel(RaycastArgs a)
+{
+    typedef __attribute__((address_space(4))) const unsigned ConstCount;
+    typedef __attribute__((address_space(4))) const DeferredRay ConstRay;
+    const unsigned total = *(ConstCount*)a.count;
+    const unsigned n = total < a.cap ? total : a.cap;
+    const float* pose = a.dyn ? a.dyn->pose : a.pose;
+    for (unsigned r = blockIdx.x; r < n; r += gridDim.x) {
+        ConstRay& cr = ((ConstRay*)a.rays)[r];
+        DeferredRay d;
+        d.job = cr.job;
+        d.px = cr.px;
+        d.guard = cr.guard;
+        d.flags = cr.flags;
+        d.t = cr.t;
+        d.prev = cr.prev;
+        d.prev_t = cr.prev_t;
+        finish_ray<FMT>(a.vol, pose, a.lv[d.job], d);
+    }
 }
 
 // Field-wise copy of a job read through the constant address space.
@@ -443,8 +764,10 @@ __device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, Raycas
 constexpr int kClaim = 2;  // tiles per claim
 
 template <int FMT>
-__global__ __launch_bounds__(64, 7) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+__global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
 {
+    const RayQueue rq{hd->ray_list + (FMT == KF_VOXEL_F32 ? 0u : hd->ray_list_cap),
+                      hd->ray_next + (FMT == KF_VOXEL_F32 ? 16 : 17), hd->ray_list_cap};
     const int nf = hd->n_ray_f32, tf = hd->ray_tiles_f32;
     const int j0 = FMT == KF_VOXEL_F32 ? 0 : nf, j1 = FMT == KF_VOXEL_F32 ? nf : hd->n_ray;
     const int t0 = FMT == KF_VOXEL_F32 ? 0 : tf, t1 = FMT == KF_VOXEL_F32 ? tf : hd->ray_tiles;
@@ -481,9 +804,40 @@ __global__ __launch_bounds__(64, 7) void raycast_batch_kernel(const BatchHeader*
                 const int lt = t - L.tile0;
                 const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
                 const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-                raycast_pixel<FMT>(vol, pose, L, i, j, i < L.w && j < L.h);
+                raycast_pixel<FMT>(vol, pose, L, i, j, i < L.w && j < L.h, rq, jb);
             }
         }
+    }
+}
+
+// The finishing pass of one voxel format over the rays its first pass handed
+// over (each record names its job).
+template <int FMT>
+__global__ __launch_bounds__(64) void raycast_batch_finish_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+{
+    typedef __attribute__((address_space(4))) const RaycastJob ConstJob;
+    typedef __attribute__((address_space(4))) const BatchHeader ConstHeader;
+    typedef __attribute__((address_space(4))) const unsigned ConstCount;
+    typedef __attribute__((address_space(4))) const DeferredRay ConstRay;
+    ConstJob* cj = (ConstJob*)jobs;
+    ConstHeader* ch = (ConstHeader*)hd;
+    const unsigned cap = ch->ray_list_cap;
+    const unsigned total = *(ConstCount*)(ch->ray_next + (FMT == KF_VOXEL_F32 ? 16 : 17));
+    const unsigned n = total < cap ? total : cap;
+    ConstRay* rays = (ConstRay*)(ch->ray_list + (FMT == KF_VOXEL_F32 ? 0u : cap));
+    for (unsigned r = blockIdx.x; r < n; r += gridDim.x) {
+        DeferredRay d;
+        d.job = rays[r].job;
+        d.px = rays[r].px;
+        d.guard = rays[r].guard;
+        d.flags = rays[r].flags;
+        d.t = rays[r].t;
+        d.prev = rays[r].prev;
+        d.prev_t = rays[r].prev_t;
+        RaycastVolume vol;
+        RaycastLevel L;
+        copy_job(cj[d.job], vol, L);
+        finish_ray<FMT>(vol, cj[d.job].pose, L, d);
     }
 }
 
@@ -508,8 +862,31 @@ int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned
 {
     raycast_batch_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
+    raycast_batch_finish_kernel<KF_VOXEL_F32><<<kFinishWaves, 64, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("raycast_batch_finish_kernel");
     raycast_batch_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
+    raycast_batch_finish_kernel<KF_VOXEL_SNORM16_U8><<<kFinishWaves, 64, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("raycast_batch_finish_kernel");
+    return KF_OK;
+}
+
+// The volume's deferred-ray list, grown to hold every ray of a launch of
+// `rays` rays (grown outside the stream order: waits for work that may still
+// use the old list).
+static int ensure_ray_list(const kf_volume* v, size_t rays, hipStream_t s)
+{
+    if (!v->ray_count) {
+        KF_HIP(hipMalloc(&v->ray_count, 2 * sizeof(unsigned)));
+        KF_HIP(hipMemset(v->ray_count, 0, 2 * sizeof(unsigned)));
+    }
+    if (v->ray_cap >= rays) return KF_OK;
+    KF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(v->ray_list);
+    v->ray_list = nullptr;
+    v->ray_cap = 0;
+    KF_HIP(hipMalloc(&v->ray_list, sizeof(DeferredRay) * rays));
+    v->ray_cap = rays;
     return KF_OK;
 }
 
@@ -517,6 +894,10 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
                           const float (*inv_k)[9], float4* const* vout, float4* const* nout, int w0, int h0,
                           hipStream_t s)
 {
+    size_t rays = 0;
+    for (int l = 0; l < levels; ++l) rays += (size_t)(w0 >> l) * (size_t)(h0 >> l);
+    int st = ensure_ray_list(v, rays, s);
+    if (st) return st;
     RaycastArgs a{};
     a.vol = raycast_volume(v);
     if (pose) memcpy(a.pose, pose, sizeof(a.pose));
@@ -535,12 +916,22 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
         tiles += L.tiles_x * ((L.h + 7) / 8);
     }
     a.tiles = tiles;
+    const int parity = (int)(v->ray_launches++ & 1);
+    a.rays = static_cast<DeferredRay*>(v->ray_list);
+    a.count = v->ray_count + parity;
+    a.clear = v->ray_count + (1 - parity);
+    a.cap = (unsigned)v->ray_cap;
     const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
-    if (v->format == KF_VOXEL_F32)
+    if (v->format == KF_VOXEL_F32) {
         raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
-    else
+        KF_LAUNCHED("raycast_kernel");
+        raycast_finish_kernel<KF_VOXEL_F32><<<kFinishWaves, 64, 0, s>>>(a);
+    } else {
         raycast_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(a);
-    KF_LAUNCHED("raycast_kernel");
+        KF_LAUNCHED("raycast_kernel");
+        raycast_finish_kernel<KF_VOXEL_SNORM16_U8><<<kFinishWaves, 64, 0, s>>>(a);
+    }
+    KF_LAUNCHED("raycast_finish_kernel");
     return KF_OK;
 }
 

@@ -742,6 +742,7 @@ int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, 
     if (!out) return fail(KF_ERR_ARG, "kf_volume_create: null out");
     *out = nullptr;
     if (n < 8 || n % 8 != 0) return fail(KF_ERR_ARG, "kf_volume_create: n must be a positive multiple of 8");
+    if (n > 1024) return fail(KF_ERR_ARG, "kf_volume_create: n must be at most 1024");
     if (!(width_m > 0.0f) || !(width_m < 1e4f)) return fail(KF_ERR_ARG, "kf_volume_create: width must be in (0, 1e4) m");
     // the raycast marches at >= 0.1 * mue; 1e-4 m keeps every march finite and short
     if (!(mue_m >= 1e-4f)) return fail(KF_ERR_ARG, "kf_volume_create: mue must be >= 1e-4 m");
@@ -825,6 +826,8 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->super_key);
     (void)hipFree(v->list_count);
     (void)hipFree(v->dmax);
+    (void)hipFree(v->ray_list);
+    (void)hipFree(v->ray_count);
     delete v;
     return KF_OK;
 }
