@@ -699,16 +699,31 @@ __device__ __forceinline__ int level_flag(const int* v, int level)
 // the carried state for the next launch.  Returns (via run) whether the
 // session iterates at `level` in this launch.  Ends with a barrier.
 template <bool COMP>
-__device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* cout, const float* pin, int level,
-                                int blocks, bool first, float stop_angle, float stop_dist, float* pose, int* run,
-                                double* tot, double* count)
+__device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* cout, const float* pin, int prev_blocks,
+                                int level, int blocks, bool first, float stop_angle, float stop_dist, float* pose,
+                                int* run, double* tot, double* count)
 {
-    if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
+    // The carried state (one word per lane of wave 0) and the previous pose
+    // (lanes 64..75) are loaded while the rows are summed; the rows are summed
+    // whether or not the previous launch left a pending iteration (prev_blocks
+    // is that launch's row count, known to the host; a sum nobody needs is
+    // discarded), so no load waits for the carry.
+    __shared__ IcpCarry carry;
+    constexpr int kWords = (int)(sizeof(IcpCarry) / sizeof(int));
+    int cw = 0;
+    float pw = 0.0f;
+    if (threadIdx.x < kWords) cw = reinterpret_cast<const int*>(cin)[threadIdx.x];
+    else if (threadIdx.x >= 64 && threadIdx.x < 76)
+        pw = threadIdx.x < 73 ? st->Rp[threadIdx.x - 64] : st->tp[threadIdx.x - 73];
+    if (prev_blocks > 0) sum_partials<COMP>(pin, prev_blocks, prev_blocks, tot, count);
+    if (threadIdx.x < kWords) reinterpret_cast<int*>(&carry)[threadIdx.x] = cw;
+    else if (threadIdx.x >= 64 && threadIdx.x < 76) pose[12 + (threadIdx.x - 64)] = pw;
+    __syncthreads();
     if (threadIdx.x < 64) {
         double xd[6];
-        const int rc = solve_pending(cin, tot, xd);
+        const int rc = (carry.pending && carry.status == 0) ? chol6_wave(tot, xd) : 0;
         if (threadIdx.x == 0) {
-        IcpCarry c = *cin;
+        IcpCarry c = carry;
         apply_solution(c, rc, xd, tot, *count, st, first, stop_angle, stop_dist);
         const int go = (c.status == 0 && !level_flag(c.level_done, level)) ? 1 : 0;
         c.pending = go;
@@ -716,8 +731,6 @@ __device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* co
         c.pending_blocks = blocks;
         for (int q = 0; q < 9; ++q) pose[q] = c.R[q];
         for (int q = 0; q < 3; ++q) pose[9 + q] = c.t[q];
-        for (int q = 0; q < 9; ++q) pose[12 + q] = st->Rp[q];
-        for (int q = 0; q < 3; ++q) pose[21 + q] = st->tp[q];
         *run = go;
         if (first) *cout = c;
         }
@@ -727,7 +740,8 @@ __device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* co
 
 template <bool COMP>
 __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const IcpCarry* cin, IcpCarry* cout,
-                                                            const float* pin, float stop_angle, float stop_dist)
+                                                            const float* pin, int prev_blocks, float stop_angle,
+                                                            float stop_dist)
 {
     __shared__ float pose[24];
     __shared__ int run;
@@ -737,8 +751,8 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
     LaneLoads ll;
     issue_lane_loads(a, ll, blockIdx.x);
     // phase B: finish the previous iteration
-    begin_iteration<COMP>(a.st, cin, cout, pin, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle, stop_dist,
-                          pose, &run, tot, &count);
+    begin_iteration<COMP>(a.st, cin, cout, pin, prev_blocks, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle,
+                          stop_dist, pose, &run, tot, &count);
     if (!run) return;
     // phase C: live normals, association + reduction of this iteration
     LanePixels lp;
@@ -992,7 +1006,7 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
                          int* launches)
 {
     const float sa = icp->cfg.iteration_stop_thresh_angle, sd = icp->cfg.iteration_stop_thresh_distance;
-    int k = 0;
+    int k = 0, prev_blocks = 0;  // row count of the previous launch (0: none this frame)
     for (int l = n_levels - 1; l >= 0; --l)
         for (int it = 0; it < icp->cfg.iters_per_layer[l]; ++it, ++k) {
             IcpArgs a = level_args(icp, levels[l], l);
@@ -1001,12 +1015,13 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
             if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
             if (icp->cfg.compensated)
                 icp_step_kernel<true><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
-                                                                 partials_of(icp, k + 1), sa, sd);
+                                                                 partials_of(icp, k + 1), prev_blocks, sa, sd);
             else
                 icp_step_kernel<false><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
-                                                                  partials_of(icp, k + 1), sa, sd);
+                                                                  partials_of(icp, k + 1), prev_blocks, sa, sd);
             KF_LAUNCHED("icp_step_kernel");
             if (events) KF_HIP(hipEventRecord(events[k], s));
+            prev_blocks = blocks;
         }
     if (icp->cfg.compensated)
         icp_commit_kernel<true><<<1, kThreads, 0, s>>>(icp->state, carry_of(icp, k), partials_of(icp, k + 1), sa, sd);
