@@ -707,7 +707,7 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
 
 // Workgroups (one wave each) of the finishing pass; each takes every
 // kFinishWaves-th handed-over ray.
-constexpr unsigned kFinishWaves = 4096;
+constexpr unsigned kFinishWaves = 8192;
 
 template <int FMT>
 __global__ __launch_bounds__(64) void raycast_finish_kernel(RaycastArgs a)
