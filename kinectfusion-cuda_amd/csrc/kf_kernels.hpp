@@ -111,6 +111,7 @@ struct RaycastVolume {
     const int* super_key;    // per-super-brick lower bound of F, order_key form
     int n, nb, ns, format;
     float width, mue;
+    float inv_width;  // 1 / width when width is a power of two (then x * inv_width == x / width), else 0
 };
 struct RaycastLevel {
     float Kinv[9];

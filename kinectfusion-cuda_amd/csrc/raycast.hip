@@ -55,6 +55,14 @@ __device__ __forceinline__ float voxel_f(const void* f, int64_t q)
     return snorm_decode(static_cast<const int16_t*>(f)[q]);
 }
 
+// Normalized texture coordinate x / width + 1/2 of a world coordinate.  For a
+// power-of-two width the quotient is the product with the exact reciprocal
+// (the same real number, so the same rounding), which skips the division.
+__device__ __forceinline__ float tex(const RaycastVolume& a, float x)
+{
+    return (a.inv_width != 0.0f ? x * a.inv_width : x / a.width) + 0.5f;
+}
+
 // Texel coordinates of a trilinear sample at normalized (u, v, s):
 // x = u*n - 1/2, i0 = floor(x) clamped to [0, n-1], i1 = i0 + 1 clamped.
 struct Stencil {
@@ -227,7 +235,7 @@ __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir
                                            (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
                                : 0;
     if (fs == 1) return 1;
-    const Stencil st = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+    const Stencil st = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
     if (fs < 0 && skippable(a, st)) return 1;
     f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
     return 0;
@@ -317,7 +325,7 @@ __device__ __forceinline__ void write_surface(const RaycastVolume& a, const Rayc
         const float width = a.width;
         const v3 vtx = add3(g.o, mul3s(g.dir, tstar));
         const float r = a.mue * 0.5f / width;
-        const float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
+        const float u = tex(a, vtx.x), v = tex(a, vtx.y), s = tex(a, vtx.z);
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll 1
         for (int ax = 0; ax < 3; ++ax) {
@@ -354,7 +362,7 @@ __device__ __forceinline__ void write_surface_wave(const RaycastVolume& a, const
         const float width = a.width;
         const v3 vtx = add3(g.o, mul3s(g.dir, tstar));
         const float r = a.mue * 0.5f / width;
-        const float u = vtx.x / width + 0.5f, v = vtx.y / width + 0.5f, s = vtx.z / width + 0.5f;
+        const float u = tex(a, vtx.x), v = tex(a, vtx.y), s = tex(a, vtx.z);
         float val = 0.0f;
         if (lane < 6) {
             const int ax = lane >> 1;
@@ -474,7 +482,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
                                                      (p.y * invw + 0.5f) * nf - 0.5f,
                                                      (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
                                        : 0;
-            st[k] = stencil(a.n, p.x / width + 0.5f, p.y / width + 0.5f, p.z / width + 0.5f);
+            st[k] = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
             bool skip = sk == 1;
             if (!skip && a.brick_min) {
                 const Stencil& q = st[k];
@@ -581,8 +589,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
                 if (tq[k] != nearv) {
                     if (prev_skipped) {
                         const v3 pp = add3(o, mul3s(dir, prev_t));
-                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f,
-                                                pp.z / width + 0.5f);
+                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
                     }
                     tstar = tq[k] - small * prev / (f - prev);
                 }
@@ -665,7 +672,7 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
             float pk = pv;
             if (psk) {
                 const v3 pp = add3(g.o, mul3s(g.dir, pt));
-                pk = sample_tsdf<FMT>(a.f, a.n, a.nb, pp.x / width + 0.5f, pp.y / width + 0.5f, pp.z / width + 0.5f);
+                pk = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
             }
             tstar = tkk - step * pk / (fk - pk);
             break;
@@ -744,6 +751,7 @@ __device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, Raycas
     vol.format = j.vol.format;
     vol.width = j.vol.width;
     vol.mue = j.vol.mue;
+    vol.inv_width = j.vol.inv_width;
 #pragma unroll
     for (int q = 0; q < 9; ++q) L.Kinv[q] = j.L.Kinv[q];
     L.vout = j.L.vout;
@@ -855,6 +863,8 @@ RaycastVolume raycast_volume(const kf_volume* v)
     r.format = v->format;
     r.width = v->width;
     r.mue = v->mue;
+    int e = 0;
+    r.inv_width = frexpf(v->width, &e) == 0.5f ? 1.0f / v->width : 0.0f;
     return r;
 }
 

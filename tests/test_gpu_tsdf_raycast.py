@@ -50,11 +50,12 @@ def test_fuse_bit_exact(kf, oracle32, n, L, w, h):
     np.testing.assert_array_equal(Fg.view(np.uint32), F.view(np.uint32))
 
 
-def test_raycast_bit_exact(kf, oracle32):
-    n, L, mue, w, h = 128, 3.0, 0.03, 320, 240
+@pytest.mark.parametrize("L", [3.0, 4.0])  # 4 m: power-of-two width (texture coordinates by product)
+def test_raycast_bit_exact(kf, oracle32, L):
+    n, mue, w, h = 128, 0.03, 320, 240
     seq, grid, sensor, F, W, Fg, Wg, _, _ = fuse_case(kf, oracle32, 9, n, L, mue, w, h, 3)
     for lvl in range(3):
-        announce(f"raycast n {n} level {lvl}")
+        announce(f"raycast n {n} L {L} level {lvl}")
         lw, lh = w >> lvl, h >> lvl
         vm, nm = kf.GridMap(lw, lh, 4), kf.GridMap(lw, lh, 4)
         init = np.zeros((lh, lw, 4), np.float32)
