@@ -187,6 +187,41 @@ __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float
     return -1;
 }
 
+// Samples after the one at t (same step, t + k*step by repeated addition) that
+// are certainly skipped through the cached super-brick bound.  The march's
+// approximate texel coordinates are linear in t up to rounding: t stays below
+// 6 m, so each addition is off by at most 2^-22 m, and k of them by
+// k * 2^-22 m * (texels per metre); k is capped so that this stays below
+// kMargin / 2, the coordinates' own rounding is a few 1e-4 texel.  A sample
+// whose linear coordinate is 2 kMargin inside the cached super-brick (and the
+// volume) thus has a computed coordinate kMargin inside it, and fast_skip
+// would return 1 for it from the cache.  0 when the bound does not allow
+// skipping or the current sample is near the super-brick's faces.
+__device__ __forceinline__ int super_run(const RaycastVolume& a, v3 o, v3 dir, float t, float step, float invw,
+                                         float nf, int cs, float cb)
+{
+    if (cs < 0 || !(cb >= kSkipBound)) return 0;
+    const v3 p = add3(o, mul3s(dir, t));
+    const float xa = (p.x * invw + 0.5f) * nf - 0.5f, ya = (p.y * invw + 0.5f) * nf - 0.5f,
+                za = (p.z * invw + 0.5f) * nf - 0.5f;
+    const int sx = cs % a.ns, sy = (cs / a.ns) % a.ns, sz = cs / (a.ns * a.ns);
+    const float m2 = 2.0f * kMargin, top = (float)a.n - 1.0f;
+    float kmax = 20000.0f / (nf * invw);  // k * 2^-22 * texels per metre <= kMargin / 2
+    const float pos[3] = {xa, ya, za};
+    const float d[3] = {dir.x, dir.y, dir.z};
+    const int sc[3] = {sx, sy, sz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float lo = (float)(sc[ax] * kSuper) + m2;
+        const float hi = fminf((float)(sc[ax] * kSuper + kSuper - 1), top) - m2;
+        if (!(pos[ax] >= lo && pos[ax] <= hi)) return 0;
+        const float rate = d[ax] * invw * nf * step;  // texels per step
+        if (rate > 0.0f) kmax = fminf(kmax, (hi - pos[ax]) / rate);
+        else if (rate < 0.0f) kmax = fminf(kmax, (lo - pos[ax]) / rate);
+    }
+    return kmax >= 1.0f ? (int)kmax : 0;
+}
+
 // The corner values of the last trilinear cell a march read: consecutive small
 // steps (0.1 mue) often stay in one cell, and then need no loads.
 struct CellCache {
@@ -444,9 +479,17 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         if (step == large) {
             float f = 0.0f;
             if (march_sample<FMT>(a, o, dir, t, width, invw, nf, cs, cb, ck, cbk, cc, f)) {
+                // this sample and the following ones inside the same free super-brick are
+                // skipped: positions by the same additions, no coordinates computed
+                int run = super_run(a, o, dir, t, step, invw, nf, cs, cb);
                 prev_skipped = true;
                 prev_t = t;
                 t = t + step;
+                for (; run > 0 && t < farv && guard < kMaxMarchSteps; --run) {
+                    ++guard;
+                    prev_t = t;
+                    t = t + step;
+                }
                 continue;
             }
             ++work;
