@@ -119,6 +119,9 @@ __device__ __forceinline__ void comp_merge(float& s1, float& c1, float s2, float
 }
 
 template <bool COMP>
+// Adds a pixel's row to the lane's sums of A^T A (upper triangle) and A^T b.
+// The plain form fuses each product into its sum (one rounding instead of two);
+// the compensated form adds the rounded product with its error carried.
 __device__ __forceinline__ void accumulate(float* s, float* c, const float* row, float b)
 {
     int q = 0;
@@ -126,13 +129,13 @@ __device__ __forceinline__ void accumulate(float* s, float* c, const float* row,
     for (int p = 0; p < 6; ++p)
 #pragma unroll
         for (int r = p; r < 6; ++r, ++q) {
-            float v = row[p] * row[r];
-            if (COMP) comp_add(s[q], c[q], v); else s[q] = s[q] + v;
+            if (COMP) comp_add(s[q], c[q], row[p] * row[r]);
+            else s[q] = fmaf(row[p], row[r], s[q]);
         }
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-        float v = row[p] * b;
-        if (COMP) comp_add(s[21 + p], c[21 + p], v); else s[21 + p] = s[21 + p] + v;
+        if (COMP) comp_add(s[21 + p], c[21 + p], row[p] * b);
+        else s[21 + p] = fmaf(row[p], b, s[21 + p]);
     }
 }
 

@@ -57,10 +57,19 @@ __device__ __forceinline__ float voxel_f(const void* f, int64_t q)
 
 // Normalized texture coordinate x / width + 1/2 of a world coordinate.  For a
 // power-of-two width the quotient is the product with the exact reciprocal
-// (the same real number, so the same rounding), which skips the division.
+// (the same real number, so the same rounding), which skips the division; the
+// product is exact (or far below the half's ulp), so fusing it with the
+// addition rounds the same way.
 __device__ __forceinline__ float tex(const RaycastVolume& a, float x)
 {
-    return (a.inv_width != 0.0f ? x * a.inv_width : x / a.width) + 0.5f;
+    return a.inv_width != 0.0f ? fmaf(x, a.inv_width, 0.5f) : x / a.width + 0.5f;
+}
+
+// Approximate texel coordinate of a world coordinate for the skip tests only
+// (they allow kMargin of error; this is within a few 1e-5 texel).
+__device__ __forceinline__ float approx_texel(float x, float invw, float nf)
+{
+    return fmaf(fmaf(x, invw, 0.5f), nf, -0.5f);
 }
 
 // Texel coordinates of a trilinear sample at normalized (u, v, s):
@@ -202,8 +211,8 @@ __device__ __forceinline__ int super_run(const RaycastVolume& a, v3 o, v3 dir, f
 {
     if (cs < 0 || !(cb >= kSkipBound)) return 0;
     const v3 p = add3(o, mul3s(dir, t));
-    const float xa = (p.x * invw + 0.5f) * nf - 0.5f, ya = (p.y * invw + 0.5f) * nf - 0.5f,
-                za = (p.z * invw + 0.5f) * nf - 0.5f;
+    const float xa = approx_texel(p.x, invw, nf), ya = approx_texel(p.y, invw, nf),
+                za = approx_texel(p.z, invw, nf);
     const int sx = cs % a.ns, sy = (cs / a.ns) % a.ns, sz = cs / (a.ns * a.ns);
     const float m2 = 2.0f * kMargin, top = (float)a.n - 1.0f;
     float kmax = 20000.0f / (nf * invw);  // k * 2^-22 * texels per metre <= kMargin / 2
@@ -266,8 +275,8 @@ __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir
                                             float& f)
 {
     const v3 p = add3(o, mul3s(dir, t));
-    const int fs = a.brick_min ? fast_skip(a, (p.x * invw + 0.5f) * nf - 0.5f, (p.y * invw + 0.5f) * nf - 0.5f,
-                                           (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
+    const int fs = a.brick_min ? fast_skip(a, approx_texel(p.x, invw, nf), approx_texel(p.y, invw, nf),
+                                           approx_texel(p.z, invw, nf), cs, cb, ck, cbk)
                                : 0;
     if (fs == 1) return 1;
     const Stencil st = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
@@ -521,9 +530,9 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
 #pragma unroll
         for (int k = 0; k < kGroup; ++k) {
             const v3 p = add3(o, mul3s(dir, tq[k]));
-            const int sk = a.brick_min ? cached_skip(a, (p.x * invw + 0.5f) * nf - 0.5f,
-                                                     (p.y * invw + 0.5f) * nf - 0.5f,
-                                                     (p.z * invw + 0.5f) * nf - 0.5f, cs, cb, ck, cbk)
+            const int sk = a.brick_min ? cached_skip(a, approx_texel(p.x, invw, nf),
+                                                     approx_texel(p.y, invw, nf),
+                                                     approx_texel(p.z, invw, nf), cs, cb, ck, cbk)
                                        : 0;
             st[k] = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
             bool skip = sk == 1;

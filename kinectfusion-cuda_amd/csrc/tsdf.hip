@@ -134,7 +134,10 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         bool ok = pe.z >= kMinDistance && pe.z < kMaxDistance;
         int q = 0;
         if (ok) {
-            v3 pv = m3v(a.K, pe);
+            // K pe; for a pinhole K (a.cull) the zero entries add exact zeros, so
+            // (fx x + cx z, fy y + cy z, z) is the same product
+            const v3 pv = a.cull ? mk3(a.K[0] * pe.x + a.K[6] * pe.z, a.K[4] * pe.y + a.K[7] * pe.z, pe.z)
+                                 : m3v(a.K, pe);
             float px = pv.x / pv.z, py = pv.y / pv.z;
             ok = px >= 0.0f && px < (float)a.w && py >= 0.0f && py < (float)a.h;
             if (ok) q = (int)py * a.w + (int)px;
@@ -403,6 +406,7 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
     a.brick_min = j.a.brick_min;
     a.super_key = j.a.super_key;
     a.ns = j.a.ns;
+    a.cull = j.a.cull;
     a.format = j.a.format;
 }
 
