@@ -360,13 +360,14 @@ __global__ __launch_bounds__(kThreads) void icp_rows_kernel(IcpArgs a, float* A,
 // ---- FP64 6x6 Cholesky solve, unblocked, in the column order of Eigen's LLT
 // (3rd/eigen/Eigen/src/Cholesky/LLT.h:306-326), by one wave: lane r < 6
 // holds row r of A (from the 27 sums) and of L; column k is finished by its
-// lanes and its values are broadcast with readlane.  Every element is computed
-// by the same FP64 operations in the same order as the sequential form
-//   L(k,k) = sqrt(A(k,k) - sum_j<k L(k,j)^2),
-//   L(r,k) = (A(r,k) - sum_j<k L(r,j) L(k,j)) / L(k,k),
-//   y(r) = (b(r) - sum_j<r L(r,j) y(j)) / L(r,r),
-//   x(r) = (y(r) - sum_j>r L(j,r) x(j)) / L(r,r),
-// so the result is the same, while no lane holds the whole matrix (a one-lane
+// lanes and its values are broadcast with readlane:
+//   L(k,k) = sqrt(A(k,k) - sum_j<k L(k,j)^2),   i(k) = 1 / L(k,k),
+//   L(r,k) = (A(r,k) - sum_j<k L(r,j) L(k,j)) * i(k),
+//   y(r) = (b(r) - sum_j<r L(r,j) y(j)) * i(r),
+//   x(r) = (y(r) - sum_j>r L(j,r) x(j)) * i(r)
+// (multiplying by the reciprocal instead of dividing by L(r,r) moves each
+// value by at most an ulp of FP64 and takes the FP64 divisions off the
+// substitutions' dependent chain).  No lane holds the whole matrix (a one-lane
 // FP64 solve would set the register budget of every kernel that embeds it).
 // Called by all 64 lanes of a wave; returns 0 or the failing column + 1 and
 // the solution in x (the same values in every lane).
@@ -385,12 +386,15 @@ __device__ int chol6_wave(const double* tot, double* x)
 {
     const int lane = threadIdx.x & 63;
     const int r = lane < 6 ? lane : 5;  // lanes >= 6 shadow row 5; their values are never read
-    double a[6], Lr[6];
+    double a[6], Lr[6], inv[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         a[c] = tot[sym_index(min(r, c), max(r, c))];
         Lr[c] = 0.0;
     }
+    // each diagonal entry's reciprocal is taken once (one FP64 division per
+    // column); the column below it and both substitutions multiply by it, so the
+    // substitutions' twelve dependent steps are multiplications, not divisions
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         double d = a[k];
@@ -399,6 +403,7 @@ __device__ int chol6_wave(const double* tot, double* x)
         d = bcast(d, k);
         if (!(d > 0.0)) return k + 1;
         d = sqrt(d);
+        inv[k] = 1.0 / d;
         double Lk[6];
 #pragma unroll
         for (int j = 0; j < k; ++j) Lk[j] = bcast(Lr[j], k);
@@ -408,7 +413,7 @@ __device__ int chol6_wave(const double* tot, double* x)
             double v = a[k];
 #pragma unroll
             for (int j = 0; j < k; ++j) v -= Lr[j] * Lk[j];
-            Lr[k] = v / d;
+            Lr[k] = v * inv[k];
         }
     }
     double y[6];
@@ -417,14 +422,14 @@ __device__ int chol6_wave(const double* tot, double* x)
         double v = tot[21 + r];
 #pragma unroll
         for (int j = 0; j < q; ++j) v -= Lr[j] * y[j];
-        y[q] = bcast(v / Lr[r], q);
+        y[q] = bcast(v, q) * inv[q];
     }
 #pragma unroll
     for (int q = 5; q >= 0; --q) {
         double v = y[q];
 #pragma unroll
         for (int j = q + 1; j < 6; ++j) v -= bcast(Lr[q], j) * x[j];
-        x[q] = v / bcast(Lr[q], q);
+        x[q] = v * inv[q];
     }
     return 0;
 }
