@@ -361,14 +361,15 @@ __global__ __launch_bounds__(kThreads) void icp_rows_kernel(IcpArgs a, float* A,
 // (3rd/eigen/Eigen/src/Cholesky/LLT.h:306-326), by one wave: lane r < 6
 // holds row r of A (from the 27 sums) and of L; column k is finished by its
 // lanes and its values are broadcast with readlane:
-//   L(k,k) = sqrt(A(k,k) - sum_j<k L(k,j)^2),   i(k) = 1 / L(k,k),
+//   D = A(k,k) - sum_j<k L(k,j)^2,   i(k) = 1 / sqrt(D),   L(k,k) = D i(k),
 //   L(r,k) = (A(r,k) - sum_j<k L(r,j) L(k,j)) * i(k),
 //   y(r) = (b(r) - sum_j<r L(r,j) y(j)) * i(r),
 //   x(r) = (y(r) - sum_j>r L(j,r) x(j)) * i(r)
-// (multiplying by the reciprocal instead of dividing by L(r,r) moves each
-// value by at most an ulp of FP64 and takes the FP64 divisions off the
-// substitutions' dependent chain).  No lane holds the whole matrix (a one-lane
-// FP64 solve would set the register budget of every kernel that embeds it).
+// (a reciprocal square root and multiplications instead of a square root and
+// divisions by L(r,r): each value moves by a few ulp of FP64, far below the
+// FP32 pose it feeds, and the FP64 square roots and divisions leave the
+// dependent chain).  No lane holds the whole matrix (a one-lane FP64 solve
+// would set the register budget of every kernel that embeds it).
 // Called by all 64 lanes of a wave; returns 0 or the failing column + 1 and
 // the solution in x (the same values in every lane).
 __device__ __forceinline__ double bcast(double v, int lane)
@@ -402,8 +403,8 @@ __device__ int chol6_wave(const double* tot, double* x)
         for (int j = 0; j < k; ++j) d -= Lr[j] * Lr[j];
         d = bcast(d, k);
         if (!(d > 0.0)) return k + 1;
-        d = sqrt(d);
-        inv[k] = 1.0 / d;
+        inv[k] = rsqrt(d);  // 1/sqrt, within 2 ulp of FP64
+        d = d * inv[k];
         double Lk[6];
 #pragma unroll
         for (int j = 0; j < k; ++j) Lk[j] = bcast(Lr[j], k);
