@@ -515,49 +515,68 @@ __device__ void pose_error(const float* R1, const float* t1, const float* R2, co
 // Sums the partial rows (fixed order) into 27 doubles + count in LDS.  Wave w
 // owns rows q = w, w + 4, ... (7 of the 28); lane l adds blocks l, l + 64, ...
 // in that order, then the lanes combine by butterfly.  The loads of all of a
-// wave's rows for two 64-block chunks are issued before any is added, so a
-// row set costs one memory round trip per 128 blocks.
+// wave's rows for two 64-block chunks (RowChunk) are issued before any is
+// added, so a row set costs one memory round trip per 128 blocks; the first
+// chunk can be loaded by the caller ahead of other loads (sum_loaded_partials).
+constexpr int kRowsPerWave = (kTerms + 1 + 3) / 4;
+struct RowChunk {
+    float vs[2][kRowsPerWave], vc[2][kRowsPerWave];
+};
+// Needs nblocks > 0.  Lanes past the last block load the last block's entry
+// (no branch around the loads, so none waits for another); add_row_chunk
+// skips them.
 template <bool COMP>
-__device__ void sum_partials(const float* partials, int stride, int nblocks, double* tot, double* count)
+__device__ __forceinline__ void load_row_chunk(const float* partials, int stride, int nblocks, int b0, RowChunk& ch)
 {
-    constexpr int kRows = (kTerms + 1 + 3) / 4;
+    static_assert(kThreads == 256 && 3 + 4 * (kRowsPerWave - 1) == kTerms, "four waves own rows 0..kTerms");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float s[kRows], c[kRows];
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) s[k] = c[k] = 0.0f;
-    for (int b0 = 0; b0 < nblocks; b0 += 128) {
-        float vs[2][kRows], vc[2][kRows];
+    for (int h = 0; h < 2; ++h) {
+        const int b = min(b0 + 64 * h + lane, nblocks - 1);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int b = b0 + 64 * h + lane;
-#pragma unroll
-            for (int k = 0; k < kRows; ++k) {
-                const int q = wave + 4 * k;
-                const bool is_count = (q == kTerms);
-                vs[h][k] = vc[h][k] = 0.0f;
-                if (q <= kTerms && b < nblocks) {
-                    vs[h][k] = partials[(int64_t)(is_count ? kCountSlot : q) * stride + b];
-                    if (COMP && !is_count) vc[h][k] = partials[(int64_t)(kTerms + q) * stride + b];
-                }
-            }
+        for (int k = 0; k < kRowsPerWave; ++k) {
+            const int q = wave + 4 * k;
+            const bool is_count = (q == kTerms);  // wave-uniform
+            ch.vs[h][k] = partials[(int64_t)(is_count ? kCountSlot : q) * stride + b];
+            ch.vc[h][k] = (COMP && !is_count) ? partials[(int64_t)(kTerms + q) * stride + b] : 0.0f;
         }
+    }
+}
+template <bool COMP>
+__device__ __forceinline__ void add_row_chunk(int nblocks, int b0, const RowChunk& ch, float* s, float* c)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int b = b0 + 64 * h + lane;
+    for (int h = 0; h < 2; ++h) {
+        const int b = b0 + 64 * h + lane;
 #pragma unroll
-            for (int k = 0; k < kRows; ++k) {
-                const int q = wave + 4 * k;
-                if (q <= kTerms && b < nblocks) {
-                    if (COMP && q != kTerms) comp_merge(s[k], c[k], vs[h][k], vc[h][k]);
-                    else s[k] = s[k] + vs[h][k];
-                }
+        for (int k = 0; k < kRowsPerWave; ++k) {
+            const int q = wave + 4 * k;
+            if (b < nblocks) {
+                if (COMP && q != kTerms) comp_merge(s[k], c[k], ch.vs[h][k], ch.vc[h][k]);
+                else s[k] = s[k] + ch.vs[h][k];
             }
         }
     }
+}
+// `first` = load_row_chunk(partials, stride, nblocks, 0, first); nblocks > 0.
+template <bool COMP>
+__device__ void sum_loaded_partials(const float* partials, int stride, int nblocks, const RowChunk& first,
+                                    double* tot, double* count)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float s[kRowsPerWave], c[kRowsPerWave];
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) {
+    for (int k = 0; k < kRowsPerWave; ++k) s[k] = c[k] = 0.0f;
+    add_row_chunk<COMP>(nblocks, 0, first, s, c);
+    for (int b0 = 128; b0 < nblocks; b0 += 128) {
+        RowChunk ch;
+        load_row_chunk<COMP>(partials, stride, nblocks, b0, ch);
+        add_row_chunk<COMP>(nblocks, b0, ch, s, c);
+    }
+#pragma unroll
+    for (int k = 0; k < kRowsPerWave; ++k) {
         const int q = wave + 4 * k;
-        if (q > kTerms) continue;  // wave-uniform
         const bool is_count = (q == kTerms);
         float sk = s[k], ck = c[k];
 #pragma unroll
@@ -576,6 +595,13 @@ __device__ void sum_partials(const float* partials, int stride, int nblocks, dou
         }
     }
     __syncthreads();
+}
+template <bool COMP>
+__device__ void sum_partials(const float* partials, int stride, int nblocks, double* tot, double* count)
+{
+    RowChunk first = {};
+    if (nblocks > 0) load_row_chunk<COMP>(partials, stride, nblocks, 0, first);
+    sum_loaded_partials<COMP>(partials, stride, nblocks, first, tot, count);
 }
 
 // Pose update of one iteration from its FP64 solution (one lane): the float
@@ -707,26 +733,49 @@ __device__ __forceinline__ int level_flag(const int* v, int level)
 // workgroup holds the session's partial row 0: it records the solve and writes
 // the carried state for the next launch.  Returns (via run) whether the
 // session iterates at `level` in this launch.  Ends with a barrier.
+// Loads of phase B that do not depend on this launch's pose: the carried
+// state (one word per lane of wave 0), the previous pose (lanes 64..75) and
+// the first chunk of the previous launch's partial rows.  Issued before the
+// pixel loads, so waiting for them does not wait for those (loads complete in
+// issue order).
+struct BeginLoads {
+    int cw;
+    float pw;
+    RowChunk rows;
+};
 template <bool COMP>
-__device__ void begin_iteration(PoseState* st, const IcpCarry* cin, IcpCarry* cout, const float* pin, int prev_blocks,
-                                int level, int blocks, bool first, float stop_angle, float stop_dist, float* pose,
-                                int* run, double* tot, double* count)
+__device__ __forceinline__ void issue_begin_loads(const PoseState* st, const IcpCarry* cin, const float* pin,
+                                                  int prev_blocks, BeginLoads& bl)
 {
-    // The carried state (one word per lane of wave 0) and the previous pose
-    // (lanes 64..75) are loaded while the rows are summed; the rows are summed
-    // whether or not the previous launch left a pending iteration (prev_blocks
-    // is that launch's row count, known to the host; a sum nobody needs is
-    // discarded), so no load waits for the carry.
+    // every lane loads (indexes clamped; begin_iteration keeps the wanted
+    // lanes' words), so no branch makes a load wait for the loads before it
+    constexpr int kWords = (int)(sizeof(IcpCarry) / sizeof(int));
+    bl.cw = reinterpret_cast<const int*>(cin)[min((int)threadIdx.x & 63, kWords - 1)];
+    const int pi = min((int)threadIdx.x & 63, 11);
+    bl.pw = *(pi < 9 ? st->Rp + pi : st->tp + (pi - 9));
+    // no branch around the row loads either: with no previous launch
+    // (prev_blocks = 0) every lane reads pin[0], and nothing is summed
+    load_row_chunk<COMP>(pin, prev_blocks, max(prev_blocks, 1), 0, bl.rows);
+}
+
+// Phase B of a launch for one session: finish the previous iteration from
+// (cin, pin) and put the pose of this iteration in LDS.  `first` = this
+// workgroup holds the session's partial row 0: it records the solve and writes
+// the carried state for the next launch.  Returns (via run) whether the
+// session iterates at `level` in this launch.  Ends with a barrier.  The rows
+// are summed whether or not the previous launch left a pending iteration
+// (prev_blocks is that launch's row count, known to the host; a sum nobody
+// needs is discarded), so no load waits for the carry.
+template <bool COMP>
+__device__ void begin_iteration(const BeginLoads& bl, PoseState* st, IcpCarry* cout, const float* pin,
+                                int prev_blocks, int level, int blocks, bool first, float stop_angle, float stop_dist,
+                                float* pose, int* run, double* tot, double* count)
+{
     __shared__ IcpCarry carry;
     constexpr int kWords = (int)(sizeof(IcpCarry) / sizeof(int));
-    int cw = 0;
-    float pw = 0.0f;
-    if (threadIdx.x < kWords) cw = reinterpret_cast<const int*>(cin)[threadIdx.x];
-    else if (threadIdx.x >= 64 && threadIdx.x < 76)
-        pw = threadIdx.x < 73 ? st->Rp[threadIdx.x - 64] : st->tp[threadIdx.x - 73];
-    if (prev_blocks > 0) sum_partials<COMP>(pin, prev_blocks, prev_blocks, tot, count);
-    if (threadIdx.x < kWords) reinterpret_cast<int*>(&carry)[threadIdx.x] = cw;
-    else if (threadIdx.x >= 64 && threadIdx.x < 76) pose[12 + (threadIdx.x - 64)] = pw;
+    if (prev_blocks > 0) sum_loaded_partials<COMP>(pin, prev_blocks, prev_blocks, bl.rows, tot, count);
+    if (threadIdx.x < kWords) reinterpret_cast<int*>(&carry)[threadIdx.x] = bl.cw;
+    else if (threadIdx.x >= 64 && threadIdx.x < 76) pose[12 + (threadIdx.x - 64)] = bl.pw;
     __syncthreads();
     if (threadIdx.x < 64) {
         double xd[6];
@@ -756,11 +805,15 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
     __shared__ int run;
     __shared__ double tot[kTerms];
     __shared__ double count;
-    // phase A: pose-independent loads of this lane's pixels (in flight during B)
+    // phase A: the loads of phase B, then the pose-independent loads of this
+    // lane's pixels (in flight during B)
+    BeginLoads bl;
+    issue_begin_loads<COMP>(a.st, cin, pin, prev_blocks, bl);
     LaneLoads ll;
     issue_lane_loads(a, ll, blockIdx.x);
+    __builtin_amdgcn_sched_barrier(0);  // nothing of phase B moves above the pixel loads
     // phase B: finish the previous iteration
-    begin_iteration<COMP>(a.st, cin, cout, pin, prev_blocks, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle,
+    begin_iteration<COMP>(bl, a.st, cout, pin, prev_blocks, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle,
                           stop_dist, pose, &run, tot, &count);
     if (!run) return;
     // phase C: live normals, association + reduction of this iteration
