@@ -515,23 +515,27 @@ __device__ void pose_error(const float* R1, const float* t1, const float* R2, co
 // Sums the partial rows (fixed order) into 27 doubles + count in LDS.  Wave w
 // owns rows q = w, w + 4, ... (7 of the 28); lane l adds blocks l, l + 64, ...
 // in that order, then the lanes combine by butterfly.  The loads of all of a
-// wave's rows for two 64-block chunks (RowChunk) are issued before any is
-// added, so a row set costs one memory round trip per 128 blocks; the first
-// chunk can be loaded by the caller ahead of other loads (sum_loaded_partials).
+// wave's rows for a chunk of kRowHalves 64-block groups (RowChunk) are issued
+// before any is added, so a row set costs one memory round trip per 320 blocks
+// (a 640x480 level is 300); the first chunk can be loaded by the caller ahead
+// of other loads (sum_loaded_partials).
 constexpr int kRowsPerWave = (kTerms + 1 + 3) / 4;
+constexpr int kRowHalves = 5;
 struct RowChunk {
-    float vs[2][kRowsPerWave], vc[2][kRowsPerWave];
+    float vs[kRowHalves][kRowsPerWave], vc[kRowHalves][kRowsPerWave];
 };
 // Needs nblocks > 0.  Lanes past the last block load the last block's entry
-// (no branch around the loads, so none waits for another); add_row_chunk
-// skips them.
+// (no divergent branch around the loads, so none waits for another);
+// add_row_chunk skips them.  Groups wholly past the last block are not loaded
+// (a wave-uniform test).
 template <bool COMP>
 __device__ __forceinline__ void load_row_chunk(const float* partials, int stride, int nblocks, int b0, RowChunk& ch)
 {
     static_assert(kThreads == 256 && 3 + 4 * (kRowsPerWave - 1) == kTerms, "four waves own rows 0..kTerms");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kRowHalves; ++h) {
+        if (h > 0 && b0 + 64 * h >= nblocks) break;
         const int b = min(b0 + 64 * h + lane, nblocks - 1);
 #pragma unroll
         for (int k = 0; k < kRowsPerWave; ++k) {
@@ -547,7 +551,8 @@ __device__ __forceinline__ void add_row_chunk(int nblocks, int b0, const RowChun
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kRowHalves; ++h) {
+        if (h > 0 && b0 + 64 * h >= nblocks) break;
         const int b = b0 + 64 * h + lane;
 #pragma unroll
         for (int k = 0; k < kRowsPerWave; ++k) {
@@ -569,7 +574,7 @@ __device__ void sum_loaded_partials(const float* partials, int stride, int nbloc
 #pragma unroll
     for (int k = 0; k < kRowsPerWave; ++k) s[k] = c[k] = 0.0f;
     add_row_chunk<COMP>(nblocks, 0, first, s, c);
-    for (int b0 = 128; b0 < nblocks; b0 += 128) {
+    for (int b0 = 64 * kRowHalves; b0 < nblocks; b0 += 64 * kRowHalves) {
         RowChunk ch;
         load_row_chunk<COMP>(partials, stride, nblocks, b0, ch);
         add_row_chunk<COMP>(nblocks, b0, ch, s, c);
