@@ -606,7 +606,8 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
                 J.a.cos_thresh = cos_th;
                 J.a.st = q.st;
                 J.a.level = l;
-                J.a.stride = kf::icp_blocks_for(w >> l, h >> l);
+                J.a.pix = kf::icp_pix_for((int64_t)(w >> l) * (h >> l));
+                J.a.stride = kf::icp_level_blocks(w >> l, h >> l);
                 J.carry = q.carry;
                 J.partials = q.partials;
                 J.max_blocks = q.max_blocks;

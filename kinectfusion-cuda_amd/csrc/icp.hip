@@ -31,7 +31,7 @@ namespace kf {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPixPerThread = 4;
+constexpr int kPixPerThread = 4;  // pixels per lane, at most (see icp_pix_for)
 constexpr int kTerms = 27;
 // partial layout (component-major): [0,27) sums, [27,54) compensations, 54 count
 constexpr int kCountSlot = 54;
@@ -230,42 +230,52 @@ __device__ __forceinline__ void block_reduce_store(float* s_in, float* c_in, int
     }
 }
 
-// The pixels of lane (block, k, lane) and their pose-independent inputs.
+// The pixels of lane (block, k, lane), k < pix <= PIX, and their
+// pose-independent inputs: pixel (block * pix + k) * kThreads + lane.
+template <int PIX = kPixPerThread>
 struct LanePixels {
-    PixelIn in[kPixPerThread];
+    PixelIn in[PIX];
 };
 // Their vertex loads only (a pixel past the map reads as invalid), so the
 // loads can be in flight while the workgroup finishes the previous iteration.
+template <int PIX = kPixPerThread>
 struct LaneLoads {
-    NormalStencil ns[kPixPerThread];
+    NormalStencil ns[PIX];
 };
-__device__ __forceinline__ void issue_lane_loads(const IcpArgs& a, LaneLoads& ll, int block)
+template <int PIX>
+__device__ __forceinline__ void issue_lane_loads(const IcpArgs& a, LaneLoads<PIX>& ll, int block, int pix)
 {
     const int64_t npx = (int64_t)a.w * a.h;
 #pragma unroll
-    for (int k = 0; k < kPixPerThread; ++k) {
-        int64_t px = ((int64_t)block * kPixPerThread + k) * kThreads + threadIdx.x;
+    for (int k = 0; k < PIX; ++k) {
+        if (k >= pix) break;  // uniform
+        int64_t px = ((int64_t)block * pix + k) * kThreads + threadIdx.x;
         const bool live = px < npx;
         int u = live ? (int)(px % a.w) : -1, v = live ? (int)(px / a.w) : -1;
         ll.ns[k] = load_normal_stencil(a.V, a.w, a.h, u, v);
     }
 }
-__device__ __forceinline__ void finish_lane_pixels(const LaneLoads& ll, LanePixels& lp)
+template <int PIX>
+__device__ __forceinline__ void finish_lane_pixels(const LaneLoads<PIX>& ll, LanePixels<PIX>& lp, int pix)
 {
 #pragma unroll
-    for (int k = 0; k < kPixPerThread; ++k) lp.in[k] = icp_pixel_from(ll.ns[k]);
+    for (int k = 0; k < PIX; ++k) {
+        if (k >= pix) break;
+        lp.in[k] = icp_pixel_from(ll.ns[k]);
+    }
 }
-__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels& lp, int block)
+template <int PIX>
+__device__ __forceinline__ void load_lane_pixels(const IcpArgs& a, LanePixels<PIX>& lp, int block, int pix)
 {
-    LaneLoads ll;
-    issue_lane_loads(a, ll, block);
-    finish_lane_pixels(ll, lp);
+    LaneLoads<PIX> ll;
+    issue_lane_loads(a, ll, block, pix);
+    finish_lane_pixels(ll, lp, pix);
 }
 
 // pose: R (9), t (3), Rp (9), tp (3) of the iteration, in LDS.  Writes partial
 // row `block` of a.partials.
-template <bool COMP>
-__device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const float* pose, int block)
+template <bool COMP, int PIX>
+__device__ void associate_reduce(const IcpArgs& a, const LanePixels<PIX>& lp, const float* pose, int block, int pix)
 {
     const float* Rc = pose;
     const float* tc = pose + 9;
@@ -278,7 +288,8 @@ __device__ void associate_reduce(const IcpArgs& a, const LanePixels& lp, const f
     for (int q = 0; q < kTerms; ++q) s[q] = c[q] = 0.0f;
     int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < kPixPerThread; ++k) {
+    for (int k = 0; k < PIX; ++k) {
+        if (k >= pix) break;
         float row[6], b;
         if (icp_pixel_row(a, lp.in[k], Rc, tc, M, tp, row, b)) {
             accumulate<COMP>(s, c, row, b);
@@ -293,14 +304,14 @@ __device__ void reduce_body(const IcpArgs& a)
 {
     PoseState* st = a.st;
     __shared__ float pose[24];
-    LanePixels lp;
-    load_lane_pixels(a, lp, blockIdx.x);
+    LanePixels<> lp;
+    load_lane_pixels(a, lp, blockIdx.x, kPixPerThread);
     if (threadIdx.x < 9) pose[threadIdx.x] = st->R[threadIdx.x];
     else if (threadIdx.x < 12) pose[threadIdx.x] = st->t[threadIdx.x - 9];
     else if (threadIdx.x < 21) pose[threadIdx.x] = st->Rp[threadIdx.x - 12];
     else if (threadIdx.x < 24) pose[threadIdx.x] = st->tp[threadIdx.x - 21];
     __syncthreads();
-    associate_reduce<COMP>(a, lp, pose, blockIdx.x);
+    associate_reduce<COMP>(a, lp, pose, blockIdx.x, kPixPerThread);
 }
 
 template <bool COMP>
@@ -801,7 +812,7 @@ __device__ void begin_iteration(const BeginLoads& bl, PoseState* st, IcpCarry* c
     __syncthreads();
 }
 
-template <bool COMP>
+template <bool COMP, int PIX>
 __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const IcpCarry* cin, IcpCarry* cout,
                                                             const float* pin, int prev_blocks, float stop_angle,
                                                             float stop_dist)
@@ -814,17 +825,17 @@ __global__ __launch_bounds__(kThreads) void icp_step_kernel(IcpArgs a, const Icp
     // lane's pixels (in flight during B)
     BeginLoads bl;
     issue_begin_loads<COMP>(a.st, cin, pin, prev_blocks, bl);
-    LaneLoads ll;
-    issue_lane_loads(a, ll, blockIdx.x);
+    LaneLoads<PIX> ll;
+    issue_lane_loads(a, ll, blockIdx.x, PIX);
     __builtin_amdgcn_sched_barrier(0);  // nothing of phase B moves above the pixel loads
     // phase B: finish the previous iteration
     begin_iteration<COMP>(bl, a.st, cout, pin, prev_blocks, a.level, (int)gridDim.x, blockIdx.x == 0, stop_angle,
                           stop_dist, pose, &run, tot, &count);
     if (!run) return;
     // phase C: live normals, association + reduction of this iteration
-    LanePixels lp;
-    finish_lane_pixels(ll, lp);
-    associate_reduce<COMP>(a, lp, pose, blockIdx.x);
+    LanePixels<PIX> lp;
+    finish_lane_pixels(ll, lp, PIX);
+    associate_reduce<COMP>(a, lp, pose, blockIdx.x, PIX);
 }
 
 // Starts a frame's ICP: the working and previous poses are the sensor pose
@@ -1005,9 +1016,10 @@ __global__ __launch_bounds__(kThreads) void icp_batch_assoc_kernel(const BatchHe
         a.level = level;
         a.partials = cj[j].partials;
         a.stride = cj[j].a.stride;
+        a.pix = cj[j].a.pix;
         const int lb = b - cj[j].block0;
-        LanePixels lp;
-        load_lane_pixels(a, lp, lb);
+        LanePixels<> lp;
+        load_lane_pixels(a, lp, lb, a.pix);
         if (j != cur) {
             cur = j;
             __syncthreads();  // the previous session's pose is no longer read
@@ -1025,15 +1037,40 @@ __global__ __launch_bounds__(kThreads) void icp_batch_assoc_kernel(const BatchHe
             __syncthreads();  // the previous block's reduction LDS is free again
         }
         if (!run) continue;
-        associate_reduce<COMP>(a, lp, pose, lb);
+        associate_reduce<COMP>(a, lp, pose, lb, a.pix);
     }
 }
 
-inline int blocks_for(int64_t n) { return (int)((n + kThreads * kPixPerThread - 1) / (kThreads * kPixPerThread)); }
+inline int blocks_for(int64_t n, int pix = kPixPerThread)
+{
+    return (int)((n + (int64_t)kThreads * pix - 1) / ((int64_t)kThreads * pix));
+}
 
 }  // namespace
 
-int icp_blocks_for(int w, int h) { return blocks_for((int64_t)w * h); }
+// Pixels per lane of an iteration launch over an n-pixel map: the most (4, 2
+// or 1) that still gives kFillBlocks workgroups, else 1.  An iteration's
+// association is one lane's pixels in sequence, so a coarse level spread over
+// more workgroups ends sooner.  A function of n alone: the batch and the
+// single-session loop choose alike and sum the same partial rows.
+int icp_pix_for(int64_t n)
+{
+    constexpr int kFillBlocks = 256;
+    for (int pix = kPixPerThread; pix > 1; pix >>= 1)
+        if (blocks_for(n, pix) >= kFillBlocks) return pix;
+    return 1;
+}
+int icp_level_blocks(int w, int h)
+{
+    const int64_t n = (int64_t)w * h;
+    return blocks_for(n, icp_pix_for(n));
+}
+int icp_blocks_for(int w, int h)
+{
+    int m = 0;
+    for (int l = 0; l < 3; ++l) m = std::max(m, icp_level_blocks(std::max(w >> l, 1), std::max(h >> l, 1)));
+    return m;
+}
 
 static IcpCarry* carry_of(kf_icp* icp, int k) { return reinterpret_cast<IcpCarry*>(icp->carry) + (k & 1); }
 static float* partials_of(kf_icp* icp, int k)
@@ -1063,7 +1100,8 @@ static IcpArgs level_args(const kf_icp* icp, const IcpLevel& L, int l)
     a.cos_thresh = icp->cos_thresh;
     a.st = icp->state;
     a.level = l;
-    a.stride = blocks_for((int64_t)L.w * L.h);
+    a.pix = icp_pix_for((int64_t)L.w * L.h);
+    a.stride = blocks_for((int64_t)L.w * L.h, a.pix);
     return a;
 }
 
@@ -1080,12 +1118,12 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
             a.partials = partials_of(icp, k);
             const int blocks = a.stride;
             if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
-            if (icp->cfg.compensated)
-                icp_step_kernel<true><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
-                                                                 partials_of(icp, k + 1), prev_blocks, sa, sd);
-            else
-                icp_step_kernel<false><<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1),
-                                                                  partials_of(icp, k + 1), prev_blocks, sa, sd);
+            using Step = void (*)(IcpArgs, const IcpCarry*, IcpCarry*, const float*, int, float, float);
+            const Step steps[2][3] = {{icp_step_kernel<false, 1>, icp_step_kernel<false, 2>, icp_step_kernel<false, 4>},
+                                      {icp_step_kernel<true, 1>, icp_step_kernel<true, 2>, icp_step_kernel<true, 4>}};
+            const Step step = steps[icp->cfg.compensated ? 1 : 0][a.pix == 4 ? 2 : a.pix == 2 ? 1 : 0];
+            step<<<blocks, kThreads, 0, s>>>(a, carry_of(icp, k), carry_of(icp, k + 1), partials_of(icp, k + 1),
+                                             prev_blocks, sa, sd);
             KF_LAUNCHED("icp_step_kernel");
             if (events) KF_HIP(hipEventRecord(events[k], s));
             prev_blocks = blocks;

@@ -156,6 +156,7 @@ struct IcpArgs {
     int level;
     float* partials;
     int stride;     // partial-row count of this launch (or of this job)
+    int pix;        // pixels per lane (icp_pix_for of the map's pixel count)
 };
 // State carried from one ICP launch to the next (double-buffered by launch parity).
 struct IcpCarry {
@@ -184,7 +185,9 @@ struct IcpLevel {
     int w, h;
     float K[9];
 };
-int icp_blocks_for(int w, int h);
+int icp_pix_for(int64_t n);
+int icp_level_blocks(int w, int h);  // workgroups of one iteration launch over a w x h map
+int icp_blocks_for(int w, int h);    // the most of icp_level_blocks over the three levels of a w x h frame
 // Enqueues the whole coarse-to-fine loop on `s` with the state on the device,
 // ending with the commit of the estimate into state->pose / inv_pose.
 int launch_icp_loop(kf_icp* icp, const IcpLevel* levels, int n_levels, hipStream_t s);
