@@ -63,17 +63,36 @@ __device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
 
 // false = zero row (rejection reasons as icp.cu:29-70, evaluated in an order
 // that does not change the result: every rejection writes the same zero row).
-__device__ __forceinline__ bool icp_pixel_row(const IcpArgs& a, const PixelIn& in, const float* Rc, const float* tc,
-                                              const float* M, const float* tp, float* row, float& b)
+// Split in two so that the target loads of all of a lane's pixels are in
+// flight together: icp_pixel_gather transforms and projects the pixel and
+// issues the loads of its target vertex and normal (at an in-map index even
+// when the projection is off the map, so no branch surrounds the loads);
+// icp_pixel_finish applies the tests and builds the row.
+struct PixelGather {
+    v3 s;             // the vertex in the frame of the current estimate
+    float4 tv, tn;    // target vertex and normal at the projection
+    bool ok;          // the pixel is valid and projects into the map
+};
+__device__ __forceinline__ PixelGather icp_pixel_gather(const IcpArgs& a, const PixelIn& in, const float* Rc,
+                                                       const float* tc, const float* M, const float* tp)
 {
-    if (!in.ok) return false;
-    v3 s = add3(m3v(Rc, in.vc), mk3(tc[0], tc[1], tc[2]));
+    PixelGather g;
+    g.s = add3(m3v(Rc, in.vc), mk3(tc[0], tc[1], tc[2]));
     // icp.cuh:40-49: (K * R_prev^T) * (s - t_prev), floor of the perspective division
-    v3 p = m3v(M, sub3(s, mk3(tp[0], tp[1], tp[2])));
+    v3 p = m3v(M, sub3(g.s, mk3(tp[0], tp[1], tp[2])));
     float fu = floorf(p.x / p.z), fv = floorf(p.y / p.z);
-    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)a.w && fv < (float)a.h)) return false;
-    int uc = (int)fu, vcr = (int)fv;
-    const float4 tv = a.TV[(int64_t)vcr * a.w + uc], tn = a.TN[(int64_t)vcr * a.w + uc];
+    g.ok = in.ok && fu >= 0.0f && fv >= 0.0f && fu < (float)a.w && fv < (float)a.h;
+    const int64_t q = g.ok ? (int64_t)(int)fv * a.w + (int)fu : 0;
+    g.tv = a.TV[q];
+    g.tn = a.TN[q];
+    return g;
+}
+__device__ __forceinline__ bool icp_pixel_finish(const IcpArgs& a, const PixelIn& in, const PixelGather& g,
+                                                 const float* Rc, float* row, float& b)
+{
+    if (!g.ok) return false;
+    const float4 tv = g.tv, tn = g.tn;
+    const v3 s = g.s;
     if (tv.w != kValid || tn.w != kValid) return false;
     v3 d = xyz(tv);
     if (len3(sub3(d, s)) > a.dist_thresh) return false;  // icp.cuh:57-60
@@ -91,6 +110,11 @@ __device__ __forceinline__ bool icp_pixel_row(const IcpArgs& a, const PixelIn& i
     row[5] = n.z;
     b = ((((n.x * d.x + n.y * d.y) + n.z * d.z) - n.x * s.x) - n.y * s.y) - n.z * s.z;
     return true;
+}
+__device__ __forceinline__ bool icp_pixel_row(const IcpArgs& a, const PixelIn& in, const float* Rc, const float* tc,
+                                              const float* M, const float* tp, float* row, float& b)
+{
+    return icp_pixel_finish(a, in, icp_pixel_gather(a, in, Rc, tc, M, tp), Rc, row, b);
 }
 
 __device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const float* Rc, const float* tc,
@@ -287,11 +311,17 @@ __device__ void associate_reduce(const IcpArgs& a, const LanePixels<PIX>& lp, co
 #pragma unroll
     for (int q = 0; q < kTerms; ++q) s[q] = c[q] = 0.0f;
     int cnt = 0;
+    PixelGather g[PIX];
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) {
+        if (k >= pix) break;
+        g[k] = icp_pixel_gather(a, lp.in[k], Rc, tc, M, tp);
+    }
 #pragma unroll
     for (int k = 0; k < PIX; ++k) {
         if (k >= pix) break;
         float row[6], b;
-        if (icp_pixel_row(a, lp.in[k], Rc, tc, M, tp, row, b)) {
+        if (icp_pixel_finish(a, lp.in[k], g[k], Rc, row, b)) {
             accumulate<COMP>(s, c, row, b);
             ++cnt;
         }
