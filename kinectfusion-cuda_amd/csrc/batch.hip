@@ -155,6 +155,8 @@ int check_session_config(const kf_session_config* c)
 {
     if (c->frame_width <= 0 || c->frame_height <= 0 || c->frame_width % 4 || c->frame_height % 4)
         return kf::fail(KF_ERR_DIMS, "Can't construct pyramid. Dimensions must be divisible by 2^(m_n_layers - 1).");
+    if ((int64_t)c->frame_width * c->frame_height > kf::icp_max_map_pixels())
+        return kf::fail(KF_ERR_DIMS, "session: frame above 2^24 pixels");
     if (c->volume_n < 8 || c->volume_n % 8) return kf::fail(KF_ERR_ARG, "session: volume_n must be a multiple of 8");
     if (c->voxel_format != KF_VOXEL_F32 && c->voxel_format != KF_VOXEL_SNORM16_U8)
         return kf::fail(KF_ERR_ARG, "session: unknown voxel format");

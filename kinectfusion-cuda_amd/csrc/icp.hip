@@ -266,16 +266,33 @@ template <int PIX = kPixPerThread>
 struct LaneLoads {
     NormalStencil ns[PIX];
 };
+// Column and row of pixel px of a w-wide map, px < kMaxMapPixels: px is exact
+// in FP32 and px * (1/w) is within one of px / w, so one correction each way
+// gives the quotient.
+constexpr int kMaxMapPixels = 1 << 24;
+__device__ __forceinline__ void pixel_uv(int px, int w, float inv_w, int& u, int& v)
+{
+    v = (int)((float)px * inv_w);
+    u = px - v * w;
+    if (u < 0) {
+        u += w;
+        --v;
+    } else if (u >= w) {
+        u -= w;
+        ++v;
+    }
+}
 template <int PIX>
 __device__ __forceinline__ void issue_lane_loads(const IcpArgs& a, LaneLoads<PIX>& ll, int block, int pix)
 {
-    const int64_t npx = (int64_t)a.w * a.h;
+    const int npx = a.w * a.h;  // <= kMaxMapPixels (checked on the host)
+    const float inv_w = 1.0f / (float)a.w;
 #pragma unroll
     for (int k = 0; k < PIX; ++k) {
         if (k >= pix) break;  // uniform
-        int64_t px = ((int64_t)block * pix + k) * kThreads + threadIdx.x;
-        const bool live = px < npx;
-        int u = live ? (int)(px % a.w) : -1, v = live ? (int)(px / a.w) : -1;
+        const int px = (block * pix + k) * kThreads + (int)threadIdx.x;
+        int u = -1, v = -1;
+        if (px < npx) pixel_uv(px, a.w, inv_w, u, v);
         ll.ns[k] = load_normal_stencil(a.V, a.w, a.h, u, v);
     }
 }
@@ -407,9 +424,9 @@ __global__ __launch_bounds__(kThreads) void icp_rows_kernel(IcpArgs a, float* A,
 //   y(r) = (b(r) - sum_j<r L(r,j) y(j)) * i(r),
 //   x(r) = (y(r) - sum_j>r L(j,r) x(j)) * i(r)
 // (a reciprocal square root and multiplications instead of a square root and
-// divisions by L(r,r): each value moves by a few ulp of FP64, far below the
-// FP32 pose it feeds, and the FP64 square roots and divisions leave the
-// dependent chain).  No lane holds the whole matrix (a one-lane FP64 solve
+// divisions by L(r,r), and fused multiply-adds in the sums: each value moves
+// by a few ulp of FP64, far below the FP32 pose it feeds, and the FP64 square
+// roots and divisions leave the dependent chain).  No lane holds the whole matrix (a one-lane FP64 solve
 // would set the register budget of every kernel that embeds it).
 // Called by all 64 lanes of a wave; returns 0 or the failing column + 1 and
 // the solution in x (the same values in every lane).
@@ -434,14 +451,14 @@ __device__ int chol6_wave(const double* tot, double* x)
         a[c] = tot[sym_index(min(r, c), max(r, c))];
         Lr[c] = 0.0;
     }
-    // each diagonal entry's reciprocal is taken once (one FP64 division per
-    // column); the column below it and both substitutions multiply by it, so the
+    // each diagonal entry's reciprocal square root is taken once per column;
+    // the column below it and both substitutions multiply by it, so the
     // substitutions' twelve dependent steps are multiplications, not divisions
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         double d = a[k];
 #pragma unroll
-        for (int j = 0; j < k; ++j) d -= Lr[j] * Lr[j];
+        for (int j = 0; j < k; ++j) d = fma(-Lr[j], Lr[j], d);
         d = bcast(d, k);
         if (!(d > 0.0)) return k + 1;
         inv[k] = rsqrt(d);  // 1/sqrt, within 2 ulp of FP64
@@ -454,7 +471,7 @@ __device__ int chol6_wave(const double* tot, double* x)
         } else if (r > k) {
             double v = a[k];
 #pragma unroll
-            for (int j = 0; j < k; ++j) v -= Lr[j] * Lk[j];
+            for (int j = 0; j < k; ++j) v = fma(-Lr[j], Lk[j], v);
             Lr[k] = v * inv[k];
         }
     }
@@ -463,14 +480,14 @@ __device__ int chol6_wave(const double* tot, double* x)
     for (int q = 0; q < 6; ++q) {
         double v = tot[21 + r];
 #pragma unroll
-        for (int j = 0; j < q; ++j) v -= Lr[j] * y[j];
+        for (int j = 0; j < q; ++j) v = fma(-Lr[j], y[j], v);
         y[q] = bcast(v, q) * inv[q];
     }
 #pragma unroll
     for (int q = 5; q >= 0; --q) {
         double v = y[q];
 #pragma unroll
-        for (int j = q + 1; j < 6; ++j) v -= bcast(Lr[q], j) * x[j];
+        for (int j = q + 1; j < 6; ++j) v = fma(-bcast(Lr[q], j), x[j], v);
         x[q] = v * inv[q];
     }
     return 0;
@@ -482,7 +499,7 @@ __device__ int chol6_wave(const double* tot, double* x)
 // sin and cos in FP64 of an angle x: Cody-Waite reduction r = x - k pi/2 with
 // pi/2 split in a 33-bit head (k * head exact for |k| < 2^20) and a tail,
 // then the Taylor series of sin and cos on |r| <= pi/4 (terms through r^17
-// and r^18: the next terms are below 2^-60 there).  Accurate to a few ulp of
+// and r^18: the next terms are below 2^-60 there; Horner steps fused).  Accurate to a few ulp of
 // double for |x| < 2^20 pi/2, far beyond any ICP update; the library routines
 // carry a huge-argument reduction whose registers every lane of the kernel
 // would pay for.
@@ -493,37 +510,64 @@ __device__ __forceinline__ void sincos_small(double x, double* sn, double* cs)
     const double k = rint(x * 0.636619772367581343076);
     const double r = (x - k * kHead) - k * kTail;
     const double z = r * r;
-    double ps = 1.0 / 355687428096000.0;       // 1/17!
-    ps = ps * z - 1.0 / 1307674368000.0;       // 1/15!
-    ps = ps * z + 1.0 / 6227020800.0;          // 1/13!
-    ps = ps * z - 1.0 / 39916800.0;            // 1/11!
-    ps = ps * z + 1.0 / 362880.0;              // 1/9!
-    ps = ps * z - 1.0 / 5040.0;                // 1/7!
-    ps = ps * z + 1.0 / 120.0;                 // 1/5!
-    ps = ps * z - 1.0 / 6.0;                   // 1/3!
-    const double s = r + r * (z * ps);
-    double pc = 1.0 / 6402373705728000.0;      // 1/18!
-    pc = pc * z - 1.0 / 20922789888000.0;      // 1/16!
-    pc = pc * z + 1.0 / 87178291200.0;         // 1/14!
-    pc = pc * z - 1.0 / 479001600.0;           // 1/12!
-    pc = pc * z + 1.0 / 3628800.0;             // 1/10!
-    pc = pc * z - 1.0 / 40320.0;               // 1/8!
-    pc = pc * z + 1.0 / 720.0;                 // 1/6!
-    pc = pc * z - 1.0 / 24.0;                  // 1/4!
-    pc = pc * z + 0.5;                         // 1/2!
-    const double c = 1.0 - z * pc;
+    double ps = 1.0 / 355687428096000.0;            // 1/17!
+    ps = fma(ps, z, -1.0 / 1307674368000.0);        // 1/15!
+    ps = fma(ps, z, 1.0 / 6227020800.0);            // 1/13!
+    ps = fma(ps, z, -1.0 / 39916800.0);             // 1/11!
+    ps = fma(ps, z, 1.0 / 362880.0);                // 1/9!
+    ps = fma(ps, z, -1.0 / 5040.0);                 // 1/7!
+    ps = fma(ps, z, 1.0 / 120.0);                   // 1/5!
+    ps = fma(ps, z, -1.0 / 6.0);                    // 1/3!
+    const double s = fma(r, z * ps, r);
+    double pc = 1.0 / 6402373705728000.0;           // 1/18!
+    pc = fma(pc, z, -1.0 / 20922789888000.0);       // 1/16!
+    pc = fma(pc, z, 1.0 / 87178291200.0);           // 1/14!
+    pc = fma(pc, z, -1.0 / 479001600.0);            // 1/12!
+    pc = fma(pc, z, 1.0 / 3628800.0);               // 1/10!
+    pc = fma(pc, z, -1.0 / 40320.0);                // 1/8!
+    pc = fma(pc, z, 1.0 / 720.0);                   // 1/6!
+    pc = fma(pc, z, -1.0 / 24.0);                   // 1/4!
+    pc = fma(pc, z, 0.5);                           // 1/2!
+    const double c = fma(-z, pc, 1.0);
     const int q = (int)((long long)k & 3);
     *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
     *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
 }
 
-__device__ void update_pose(float* Rm, float* t, const float* x)
+// sin and cos of the three angles of an update, as update_pose uses them:
+// yaw x[1] (sh, ch), pitch x[0] (sp, cp), roll x[2] (sb, cb).
+struct SinCos3 {
+    double sh, ch, sp, cp, sb, cb;
+};
+__device__ __forceinline__ SinCos3 angle_sincos(const float* x)
 {
-    double pitch = x[0], yaw = x[1], roll = x[2];
-    double ch, sh, cp, sp, cb, sb;
-    sincos_small(yaw, &sh, &ch);
-    sincos_small(pitch, &sp, &cp);
-    sincos_small(roll, &sb, &cb);
+    SinCos3 a;
+    sincos_small(x[1], &a.sh, &a.ch);
+    sincos_small(x[0], &a.sp, &a.cp);
+    sincos_small(x[2], &a.sb, &a.cb);
+    return a;
+}
+// The same by lanes 0, 1, 2 of a wave, from the FP64 solution xd (every lane
+// of the wave calls it and receives all six values).
+__device__ __forceinline__ SinCos3 angle_sincos_wave(const double* xd)
+{
+    const int lane = threadIdx.x & 63;
+    const float ang = lane == 0 ? (float)xd[1] : (lane == 1 ? (float)xd[0] : (float)xd[2]);
+    double sn, cs;
+    sincos_small(ang, &sn, &cs);
+    SinCos3 a;
+    a.sh = bcast(sn, 0);
+    a.ch = bcast(cs, 0);
+    a.sp = bcast(sn, 1);
+    a.cp = bcast(cs, 1);
+    a.sb = bcast(sn, 2);
+    a.cb = bcast(cs, 2);
+    return a;
+}
+
+__device__ void update_pose(float* Rm, float* t, const float* x, const SinCos3& a)
+{
+    const double sh = a.sh, ch = a.ch, sp = a.sp, cp = a.cp, sb = a.sb, cb = a.cb;
     double Ri[9];
     Ri[0] = ch * cb + sh * sp * sb;  Ri[1] = sb * cp;  Ri[2] = -sh * cb + ch * sp * sb;
     Ri[3] = -ch * sb + sh * sp * cb; Ri[4] = cb * cp;  Ri[5] = sb * sh + ch * sp * cb;
@@ -653,8 +697,8 @@ __device__ void sum_partials(const float* partials, int stride, int nblocks, dou
 // Pose update of one iteration from its FP64 solution (one lane): the float
 // update x, the pose update (icp.cpp:65-87) and the early-stop test
 // (icp.cpp:45-53).  Returns 0 or KF_ERR_NONFINITE; *done = early stop.
-__device__ int pose_step(const double* xd, float* R, float* t, float stop_angle, float stop_dist, float* x,
-                         bool* done)
+__device__ int pose_step(const double* xd, const SinCos3& sc, float* R, float* t, float stop_angle,
+                         float stop_dist, float* x, bool* done)
 {
     bool finite = true;
     for (int q = 0; q < 6; ++q) {
@@ -666,7 +710,7 @@ __device__ int pose_step(const double* xd, float* R, float* t, float stop_angle,
     float Rprev[9], tprev[3], err[2];
     for (int q = 0; q < 9; ++q) Rprev[q] = R[q];
     for (int q = 0; q < 3; ++q) tprev[q] = t[q];
-    update_pose(R, t, x);
+    update_pose(R, t, x, sc);
     pose_error(Rprev, tprev, R, t, err);
     *done = err[0] < stop_angle && err[1] < stop_dist;
     return KF_OK;
@@ -710,7 +754,7 @@ __device__ void finalize_body(PoseState* st, const float* partials, int stride, 
     float Rprev[9], tprev[3], err[2];
     for (int q = 0; q < 9; ++q) Rprev[q] = st->R[q];
     for (int q = 0; q < 3; ++q) tprev[q] = st->t[q];
-    update_pose(st->R, st->t, x);
+    update_pose(st->R, st->t, x, angle_sincos(x));
     pose_error(Rprev, tprev, st->R, st->t, err);
     st->iterations = st->iterations + 1;
     if (err[0] < stop_angle && err[1] < stop_dist) st->level_done[level] = 1;
@@ -734,14 +778,15 @@ __global__ __launch_bounds__(kThreads) void icp_finalize_kernel(PoseState* st, c
 
 // Applies the solve of the pending iteration of `c` (thread 0): rc and xd from
 // chol6_wave, tot/count the sums it was solved from.
-__device__ void apply_solution(IcpCarry& c, int rc, const double* xd, const double* tot, double count,
+__device__ void apply_solution(IcpCarry& c, int rc, const double* xd, const SinCos3& sc, const double* tot,
+                               double count,
                                PoseState* st, bool record, float stop_angle, float stop_dist)
 {
     if (!c.pending || c.status != 0) return;
     float x[6] = {0, 0, 0, 0, 0, 0};
     bool done = false;
     if (rc) rc = KF_ERR_NOT_PD;
-    else rc = pose_step(xd, c.R, c.t, stop_angle, stop_dist, x, &done);
+    else rc = pose_step(xd, sc, c.R, c.t, stop_angle, stop_dist, x, &done);
     if (rc) {
         c.status = rc;
     } else {
@@ -824,11 +869,12 @@ __device__ void begin_iteration(const BeginLoads& bl, PoseState* st, IcpCarry* c
     else if (threadIdx.x >= 64 && threadIdx.x < 76) pose[12 + (threadIdx.x - 64)] = bl.pw;
     __syncthreads();
     if (threadIdx.x < 64) {
-        double xd[6];
+        double xd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         const int rc = (carry.pending && carry.status == 0) ? chol6_wave(tot, xd) : 0;
+        const SinCos3 sc = angle_sincos_wave(xd);
         if (threadIdx.x == 0) {
         IcpCarry c = carry;
-        apply_solution(c, rc, xd, tot, *count, st, first, stop_angle, stop_dist);
+        apply_solution(c, rc, xd, sc, tot, *count, st, first, stop_angle, stop_dist);
         const int go = (c.status == 0 && !level_flag(c.level_done, level)) ? 1 : 0;
         c.pending = go;
         c.pending_level = level;
@@ -940,11 +986,12 @@ __device__ void commit_body(PoseState* st, const IcpCarry* cin, const float* pin
 {
     if (cin->pending && cin->status == 0) sum_partials<COMP>(pin, cin->pending_blocks, cin->pending_blocks, tot, count);
     if (threadIdx.x >= 64) return;
-    double xd[6];
+    double xd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int rc = solve_pending(cin, tot, xd);
+    const SinCos3 sc = angle_sincos_wave(xd);
     if (threadIdx.x != 0) return;
     IcpCarry c = *cin;
-    apply_solution(c, rc, xd, tot, *count, st, true, stop_angle, stop_dist);
+    apply_solution(c, rc, xd, sc, tot, *count, st, true, stop_angle, stop_dist);
     commit_state(st, c);
 }
 
@@ -989,11 +1036,12 @@ __global__ __launch_bounds__(kThreads) void icp_batch_solve_kernel(const BatchHe
         __syncthreads();  // every lane has read the carry before lane 0 rewrites it
         if (pend) sum_partials<COMP>(J.partials, c->pending_blocks, c->pending_blocks, tot, &count);
         if (threadIdx.x < 64) {
-            double xd[6];
+            double xd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             const int rc = pend ? chol6_wave(tot, xd) : 0;
+            const SinCos3 sc = angle_sincos_wave(xd);
             if (threadIdx.x == 0) {
                 IcpCarry cc = *c;
-                apply_solution(cc, rc, xd, tot, count, st, true, stop_angle, stop_dist);
+                apply_solution(cc, rc, xd, sc, tot, count, st, true, stop_angle, stop_dist);
                 if (!final_) {
                     cc.pending = (cc.status == 0 && !level_flag(cc.level_done, level)) ? 1 : 0;
                     cc.pending_level = level;
@@ -1090,6 +1138,7 @@ int icp_pix_for(int64_t n)
         if (blocks_for(n, pix) >= kFillBlocks) return pix;
     return 1;
 }
+int64_t icp_max_map_pixels() { return kMaxMapPixels; }
 int icp_level_blocks(int w, int h)
 {
     const int64_t n = (int64_t)w * h;
@@ -1147,7 +1196,8 @@ static int enqueue_steps(kf_icp* icp, const IcpLevel* levels, int n_levels, hipS
             IcpArgs a = level_args(icp, levels[l], l);
             a.partials = partials_of(icp, k);
             const int blocks = a.stride;
-            if (blocks > icp->max_blocks) return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
+            if (blocks > icp->max_blocks || (int64_t)a.w * a.h > kMaxMapPixels)
+                return fail(KF_ERR_DIMS, "ICP: map larger than the configured width x height");
             using Step = void (*)(IcpArgs, const IcpCarry*, IcpCarry*, const float*, int, float, float);
             const Step steps[2][3] = {{icp_step_kernel<false, 1>, icp_step_kernel<false, 2>, icp_step_kernel<false, 4>},
                                       {icp_step_kernel<true, 1>, icp_step_kernel<true, 2>, icp_step_kernel<true, 4>}};
@@ -1300,6 +1350,7 @@ int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const
 {
     if (!vertex_map || !target_vertex_map || !target_normal_map || !sums_host || w <= 0 || h <= 0)
         return kf::fail(KF_ERR_ARG, "kf_icp_reduce: bad argument");
+    if ((int64_t)w * h > kf::kMaxMapPixels) return kf::fail(KF_ERR_DIMS, "kf_icp_reduce: map above 2^24 pixels");
     hipStream_t s = kf::stream_of(stream);
     const int blocks = kf::blocks_for((int64_t)w * h);
     kf::PoseState* st = nullptr;
@@ -1362,6 +1413,8 @@ int kf_icp_create(const kf_icp_config* config, kf_icp** out)
     *out = nullptr;
     if (config->levels < 1 || config->levels > 3 || config->width <= 0 || config->height <= 0)
         return kf::fail(KF_ERR_ARG, "kf_icp_create: levels must be 1..3 and sizes positive");
+    if ((int64_t)config->width * config->height > kf::icp_max_map_pixels())
+        return kf::fail(KF_ERR_DIMS, "kf_icp_create: width x height above 2^24 pixels");
     for (int l = 0; l < config->levels; ++l)
         if (config->iters_per_layer[l] < 0) return kf::fail(KF_ERR_ARG, "kf_icp_create: negative iteration count");
     kf_icp* icp = new kf_icp();

@@ -186,6 +186,7 @@ struct IcpLevel {
     float K[9];
 };
 int icp_pix_for(int64_t n);
+int64_t icp_max_map_pixels();         // largest w x h an ICP map may have
 int icp_level_blocks(int w, int h);  // workgroups of one iteration launch over a w x h map
 int icp_blocks_for(int w, int h);    // the most of icp_level_blocks over the three levels of a w x h frame
 // Enqueues the whole coarse-to-fine loop on `s` with the state on the device,
