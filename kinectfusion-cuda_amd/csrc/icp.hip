@@ -163,24 +163,69 @@ __device__ __forceinline__ void accumulate(float* s, float* c, const float* row,
     }
 }
 
+// The lane reduction exchanges values between lanes in VALU instructions
+// only (no LDS round trips): the gfx950 permlane swaps for lane bits 5 and 4,
+// and DPP moves for bits 3 to 0 (row_mirror pairs lane r of a 16-lane row
+// with 15 - r, row_half_mirror r of an 8-lane half-row with 7 - r, quad_perm
+// the lanes two and one apart).  Every pattern pairs each lane with one
+// partner whose index differs in the step's bit.
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
+template <int CTRL>
+__device__ __forceinline__ float dpp_partner(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_partner(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+// Lanes with bit log2(OFF) clear get (a, partner's a), the others (partner's
+// b, b): in every lane `lo` is the operand of the lower lane of the pair.
+template <int OFF>
+__device__ __forceinline__ void swap_pair(float a, float b, float& lo, float& hi)
+{
+    static_assert(OFF == 32 || OFF == 16, "permlane swaps exchange 32- or 16-lane blocks");
+    const auto r = OFF == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                             : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    lo = __uint_as_float(r[0]);
+    hi = __uint_as_float(r[1]);
+}
+
 // One halving step of the lane reduction: the first n/2 entries of s (and c)
-// become, on the lane with bit `off` clear, its first half plus the partner's
-// copy and, on the other lane, its second half plus the partner's copy.  The
-// operand of the lower lane goes first, which is the order in which lane 0 of
-// a butterfly (xor off) adds, so the sums equal the butterfly's bit for bit
-// with about a sixth of the lane exchanges.
-template <bool COMP, int N>
-__device__ __forceinline__ void halve_lanes(float* s, float* c, int off, bool upper)
+// become, on the lane with the step's bit clear, its first half plus the
+// partner's copy and, on the other lane, its second half plus the partner's
+// copy; the compensated merge takes the lower lane's operand first.
+template <bool COMP, int N, int OFF>
+__device__ __forceinline__ void halve_swap(float* s, float* c)
+{
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+        float lo, hi;
+        swap_pair<OFF>(s[i], s[N / 2 + i], lo, hi);
+        if (COMP) {
+            float clo, chi;
+            swap_pair<OFF>(c[i], c[N / 2 + i], clo, chi);
+            comp_merge(lo, clo, hi, chi);
+            s[i] = lo;
+            c[i] = clo;
+        } else {
+            s[i] = lo + hi;
+        }
+    }
+}
+template <bool COMP, int N, int CTRL>
+__device__ __forceinline__ void halve_dpp(float* s, float* c, bool upper)
 {
 #pragma unroll
     for (int i = 0; i < N / 2; ++i) {
         const float send = upper ? s[i] : s[N / 2 + i];
         const float keep = upper ? s[N / 2 + i] : s[i];
-        const float recv = __shfl_xor(send, off, 64);
+        const float recv = dpp_partner<CTRL>(send);
         if (COMP) {
             const float csend = upper ? c[i] : c[N / 2 + i];
             const float ckeep = upper ? c[N / 2 + i] : c[i];
-            const float crecv = __shfl_xor(csend, off, 64);
+            const float crecv = dpp_partner<CTRL>(csend);
             float ls = upper ? recv : keep, lc = upper ? crecv : ckeep;
             comp_merge(ls, lc, upper ? keep : recv, upper ? ckeep : crecv);
             s[i] = ls;
@@ -190,11 +235,65 @@ __device__ __forceinline__ void halve_lanes(float* s, float* c, int off, bool up
         }
     }
 }
+// Sum of v over the 64 lanes of the wave, in every lane (the same value in
+// every lane: each step adds the same two operands on both lanes of a pair).
+__device__ __forceinline__ float wave_sum(float v)
+{
+    float lo, hi;
+    swap_pair<32>(v, v, lo, hi);
+    v = lo + hi;
+    swap_pair<16>(v, v, lo, hi);
+    v = lo + hi;
+    v = v + dpp_partner<kDppRowMirror>(v);
+    v = v + dpp_partner<kDppRowHalfMirror>(v);
+    v = v + dpp_partner<kDppXor2>(v);
+    return v + dpp_partner<kDppXor1>(v);
+}
+// The compensated form: lane 0 holds the total (the lower lane's operand first
+// in every merge).
+__device__ __forceinline__ void wave_comp_sum(float& s, float& c)
+{
+    const int lane = threadIdx.x & 63;
+    float lo, hi, clo, chi;
+    swap_pair<32>(s, s, lo, hi);
+    swap_pair<32>(c, c, clo, chi);
+    comp_merge(lo, clo, hi, chi);
+    swap_pair<16>(lo, lo, s, hi);
+    swap_pair<16>(clo, clo, c, chi);
+    comp_merge(s, c, hi, chi);
+#define KF_DPP_MERGE(CTRL, BIT)                                                  \
+    {                                                                            \
+        const bool up = (lane & (BIT)) != 0;                                     \
+        const float rs = dpp_partner<CTRL>(s), rc = dpp_partner<CTRL>(c);        \
+        float ls = up ? rs : s, lc = up ? rc : c;                                \
+        comp_merge(ls, lc, up ? s : rs, up ? c : rc);                            \
+        s = ls;                                                                  \
+        c = lc;                                                                  \
+    }
+    KF_DPP_MERGE(kDppRowMirror, 8)
+    KF_DPP_MERGE(kDppRowHalfMirror, 4)
+    KF_DPP_MERGE(kDppXor2, 2)
+    KF_DPP_MERGE(kDppXor1, 1)
+#undef KF_DPP_MERGE
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+    const auto r32 = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    v = (int)r32[0] + (int)r32[1];
+    const auto r16 = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    v = (int)r16[0] + (int)r16[1];
+    v += dpp_partner<kDppRowMirror>(v);
+    v += dpp_partner<kDppRowHalfMirror>(v);
+    v += dpp_partner<kDppXor2>(v);
+    v += dpp_partner<kDppXor1>(v);
+    return v;
+}
 
 // Reduces the lane values of a workgroup and writes one partial row.  The 27
 // terms are halved across the lanes (28 -> 14 -> 7 -> 4 -> 2 -> 1, one zero
 // pad at 27 and one at the 7 -> 8 step): lane L ends with term
-// 14*b5 + 7*b4 + 4*b3 + 2*b2 + b1 of its bits, every pair (L, L^1) holding it.
+// 14*b5 + 7*b4 + 4*b3 + 2*b2 + b1 of its bits, every pair (L, L^1) holding it
+// (each step keeps the first half on the lane whose step bit is clear).
 template <bool COMP>
 __device__ __forceinline__ void block_reduce_store(float* s_in, float* c_in, int cnt, float* partials, int stride,
                                                    int row)
@@ -208,18 +307,18 @@ __device__ __forceinline__ void block_reduce_store(float* s_in, float* c_in, int
         c[q] = COMP ? c_in[q] : 0.0f;
     }
     s[27] = c[27] = 0.0f;
-    halve_lanes<COMP, 28>(s, c, 32, (lane & 32) != 0);
-    halve_lanes<COMP, 14>(s, c, 16, (lane & 16) != 0);
+    halve_swap<COMP, 28, 32>(s, c);
+    halve_swap<COMP, 14, 16>(s, c);
     s[7] = c[7] = 0.0f;
-    halve_lanes<COMP, 8>(s, c, 8, (lane & 8) != 0);
-    halve_lanes<COMP, 4>(s, c, 4, (lane & 4) != 0);
-    halve_lanes<COMP, 2>(s, c, 2, (lane & 2) != 0);
+    halve_dpp<COMP, 8, kDppRowMirror>(s, c, (lane & 8) != 0);
+    halve_dpp<COMP, 4, kDppRowHalfMirror>(s, c, (lane & 4) != 0);
+    halve_dpp<COMP, 2, kDppXor2>(s, c, (lane & 2) != 0);
     {
         // last step: both lanes of the pair add the same two operands, lower first
         const bool upper = (lane & 1) != 0;
-        const float recv = __shfl_xor(s[0], 1, 64);
+        const float recv = dpp_partner<kDppXor1>(s[0]);
         if (COMP) {
-            const float crecv = __shfl_xor(c[0], 1, 64);
+            const float crecv = dpp_partner<kDppXor1>(c[0]);
             float ls = upper ? recv : s[0], lc = upper ? crecv : c[0];
             comp_merge(ls, lc, upper ? s[0] : recv, upper ? c[0] : crecv);
             s[0] = ls;
@@ -228,8 +327,7 @@ __device__ __forceinline__ void block_reduce_store(float* s_in, float* c_in, int
             s[0] = s[0] + recv;
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    cnt = wave_sum(cnt);
     const int sub = 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + ((lane >> 1) & 1);
     const int q = 14 * ((lane >> 5) & 1) + 7 * ((lane >> 4) & 1) + sub;
     if ((lane & 1) == 0 && sub < 7 && q < kTerms) {
@@ -669,16 +767,8 @@ __device__ void sum_loaded_partials(const float* partials, int stride, int nbloc
         const int q = wave + 4 * k;
         const bool is_count = (q == kTerms);
         float sk = s[k], ck = c[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            float so = __shfl_xor(sk, off, 64);
-            if (COMP && !is_count) {
-                float co = __shfl_xor(ck, off, 64);
-                comp_merge(sk, ck, so, co);
-            } else {
-                sk = sk + so;
-            }
-        }
+        if (COMP && !is_count) wave_comp_sum(sk, ck);
+        else sk = wave_sum(sk);
         if (lane == 0) {
             if (is_count) *count = (double)sk;
             else tot[q] = COMP ? (double)sk + (double)ck : (double)sk;
