@@ -163,35 +163,6 @@ __device__ __forceinline__ void accumulate(float* s, float* c, const float* row,
     }
 }
 
-// The lane reduction exchanges values between lanes in VALU instructions
-// only (no LDS round trips): the gfx950 permlane swaps for lane bits 5 and 4,
-// and DPP moves for bits 3 to 0 (row_mirror pairs lane r of a 16-lane row
-// with 15 - r, row_half_mirror r of an 8-lane half-row with 7 - r, quad_perm
-// the lanes two and one apart).  Every pattern pairs each lane with one
-// partner whose index differs in the step's bit.
-constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
-template <int CTRL>
-__device__ __forceinline__ float dpp_partner(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ int dpp_partner(int v)
-{
-    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
-}
-// Lanes with bit log2(OFF) clear get (a, partner's a), the others (partner's
-// b, b): in every lane `lo` is the operand of the lower lane of the pair.
-template <int OFF>
-__device__ __forceinline__ void swap_pair(float a, float b, float& lo, float& hi)
-{
-    static_assert(OFF == 32 || OFF == 16, "permlane swaps exchange 32- or 16-lane blocks");
-    const auto r = OFF == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
-                             : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-    lo = __uint_as_float(r[0]);
-    hi = __uint_as_float(r[1]);
-}
-
 // One halving step of the lane reduction: the first n/2 entries of s (and c)
 // become, on the lane with the step's bit clear, its first half plus the
 // partner's copy and, on the other lane, its second half plus the partner's
@@ -235,20 +206,6 @@ __device__ __forceinline__ void halve_dpp(float* s, float* c, bool upper)
         }
     }
 }
-// Sum of v over the 64 lanes of the wave, in every lane (the same value in
-// every lane: each step adds the same two operands on both lanes of a pair).
-__device__ __forceinline__ float wave_sum(float v)
-{
-    float lo, hi;
-    swap_pair<32>(v, v, lo, hi);
-    v = lo + hi;
-    swap_pair<16>(v, v, lo, hi);
-    v = lo + hi;
-    v = v + dpp_partner<kDppRowMirror>(v);
-    v = v + dpp_partner<kDppRowHalfMirror>(v);
-    v = v + dpp_partner<kDppXor2>(v);
-    return v + dpp_partner<kDppXor1>(v);
-}
 // The compensated form: lane 0 holds the total (the lower lane's operand first
 // in every merge).
 __device__ __forceinline__ void wave_comp_sum(float& s, float& c)
@@ -275,18 +232,6 @@ __device__ __forceinline__ void wave_comp_sum(float& s, float& c)
     KF_DPP_MERGE(kDppXor2, 2)
     KF_DPP_MERGE(kDppXor1, 1)
 #undef KF_DPP_MERGE
-}
-__device__ __forceinline__ int wave_sum(int v)
-{
-    const auto r32 = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
-    v = (int)r32[0] + (int)r32[1];
-    const auto r16 = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
-    v = (int)r16[0] + (int)r16[1];
-    v += dpp_partner<kDppRowMirror>(v);
-    v += dpp_partner<kDppRowHalfMirror>(v);
-    v += dpp_partner<kDppXor2>(v);
-    v += dpp_partner<kDppXor1>(v);
-    return v;
 }
 
 // Reduces the lane values of a workgroup and writes one partial row.  The 27
@@ -768,7 +713,7 @@ __device__ void sum_loaded_partials(const float* partials, int stride, int nbloc
         const bool is_count = (q == kTerms);
         float sk = s[k], ck = c[k];
         if (COMP && !is_count) wave_comp_sum(sk, ck);
-        else sk = wave_sum(sk);
+        else sk = wave_reduce(sk, [](float x, float y) { return x + y; });  // both lanes of a pair: same sum
         if (lane == 0) {
             if (is_count) *count = (double)sk;
             else tot[q] = COMP ? (double)sk + (double)ck : (double)sk;

@@ -164,6 +164,71 @@ __device__ __forceinline__ float4 compute_normal(const float4* v, int w, int h, 
 }
 
 // ---------------------------------------------------------------------------
+// Lane exchanges within a wave in VALU instructions only (no LDS round trip):
+// the gfx950 permlane swaps for lane bits 5 and 4, and DPP moves for bits 3
+// to 0 (row_mirror pairs lane r of a 16-lane row with 15 - r, row_half_mirror
+// r of an 8-lane half-row with 7 - r, quad_perm the lanes two and one apart).
+// Every pattern pairs each lane with one partner whose index differs in the
+// step's bit, so six steps combine all 64 lanes.
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
+template <int CTRL>
+__device__ __forceinline__ float dpp_partner(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_partner(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+// Lanes with bit log2(OFF) clear get (a, partner's a), the others (partner's
+// b, b): in every lane `lo` is the operand of the lower lane of the pair.
+template <int OFF>
+__device__ __forceinline__ void swap_pair(float a, float b, float& lo, float& hi)
+{
+    static_assert(OFF == 32 || OFF == 16, "permlane swaps exchange 32- or 16-lane blocks");
+    const auto r = OFF == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                             : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    lo = __uint_as_float(r[0]);
+    hi = __uint_as_float(r[1]);
+}
+
+// Wave-wide min, max and integer sum (order-free), in every lane.
+template <class Op>
+__device__ __forceinline__ float wave_reduce(float v, Op op)
+{
+    float lo, hi;
+    swap_pair<32>(v, v, lo, hi);
+    v = op(lo, hi);
+    swap_pair<16>(v, v, lo, hi);
+    v = op(lo, hi);
+    v = op(v, dpp_partner<kDppRowMirror>(v));
+    v = op(v, dpp_partner<kDppRowHalfMirror>(v));
+    v = op(v, dpp_partner<kDppXor2>(v));
+    return op(v, dpp_partner<kDppXor1>(v));
+}
+__device__ __forceinline__ float wave_min(float v)
+{
+    return wave_reduce(v, [](float a, float b) { return fminf(a, b); });
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    return wave_reduce(v, [](float a, float b) { return fmaxf(a, b); });
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+    const auto r32 = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    v = (int)r32[0] + (int)r32[1];
+    const auto r16 = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    v = (int)r16[0] + (int)r16[1];
+    v += dpp_partner<kDppRowMirror>(v);
+    v += dpp_partner<kDppRowHalfMirror>(v);
+    v += dpp_partner<kDppXor2>(v);
+    v += dpp_partner<kDppXor1>(v);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
 // TSDF volume layout in HBM: 8x8x8 bricks of 512 voxels, bricks ordered x then
 // y then z, voxels inside a brick ordered x then y then z.  A brick is 2 KiB of
 // FP32 (one plane), so a wave's 64 lanes touch whole 128-byte lines when they

@@ -144,8 +144,7 @@ __device__ __forceinline__ void measure_tile(const PreJob& J, int lt, BilateralT
         J.d0[q] = d;
         J.v[0][q] = vertex_at(d, i, j, J.Kinv[0]);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) dv = fmaxf(dv, __shfl_xor(dv, off, 64));
+    dv = kf::wave_max(dv);  // (wave_max below is the LDS array)
     if ((t & 63) == 0) wave_max[t >> 6] = dv;
     __syncthreads();
     if (t == 0) J.dmax[lt] = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));

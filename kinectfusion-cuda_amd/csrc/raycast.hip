@@ -846,7 +846,7 @@ __global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd
         for (;;) {
             int c = 0;
             if (threadIdx.x == 0) c = (int)atomicAdd(&next[band], 1u);
-            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+            c = __builtin_amdgcn_readlane(c, 0);
             const int tc = b0 + c * kClaim;
             if (tc >= b1) break;
             for (int t = tc; t < min(b1, tc + kClaim); ++t) {

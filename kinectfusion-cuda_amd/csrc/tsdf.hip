@@ -91,8 +91,11 @@ __global__ __launch_bounds__(256) void depth_max_kernel(const float* __restrict_
                 const float d = depth[(int64_t)y * w + x];
                 if (is_depth_valid(d)) m = fmaxf(m, d);
             }
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    // max over the 16 lanes of the tile (one 16-lane row)
+    m = fmaxf(m, dpp_partner<kDppRowMirror>(m));
+    m = fmaxf(m, dpp_partner<kDppRowHalfMirror>(m));
+    m = fmaxf(m, dpp_partner<kDppXor2>(m));
+    m = fmaxf(m, dpp_partner<kDppXor1>(m));
     if (sub == 0) {
         part[q] = m;
         if (tx < tx16 && ty < ty16) dmax[ty * tx16 + tx] = m;
@@ -263,8 +266,7 @@ __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, in
     float fmin = __int_as_float(0x7f800000);
     const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane, fmin);
     // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) fmin = fminf(fmin, __shfl_xor(fmin, off, 64));
+    fmin = wave_min(fmin);
     if (lane == 0 && fmin < bmin) {
         a.brick_min[brick] = fmin;
         const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
@@ -278,8 +280,7 @@ __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, in
 __device__ __forceinline__ void add_touched(unsigned long long* touched, int cnt, int* wave_cnt)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    cnt = wave_sum(cnt);
     if (lane == 0) wave_cnt[wave] = cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -538,8 +539,7 @@ __global__ __launch_bounds__(256) void brick_min_kernel(const void* f, float* br
                                               : snorm_decode(static_cast<const int16_t*>(f)[idx]);
         m = fminf(m, v);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
+    m = wave_min(m);
     if (lane == 0) {
         brick_min[brick] = m;
         const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / ((int64_t)nb * nb));
