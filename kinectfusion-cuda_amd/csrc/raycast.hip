@@ -370,17 +370,10 @@ __device__ __forceinline__ void write_surface(const RaycastVolume& a, const Rayc
         const v3 vtx = add3(g.o, mul3s(g.dir, tstar));
         const float r = a.mue * 0.5f / width;
         const float u = tex(a, vtx.x), v = tex(a, vtx.y), s = tex(a, vtx.z);
-        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-#pragma unroll 1
-        for (int ax = 0; ax < 3; ++ax) {
-            const float du = ax == 0 ? r : 0.0f, dv = ax == 1 ? r : 0.0f, ds = ax == 2 ? r : 0.0f;
-            const float fp = sample_tsdf<FMT>(a.f, a.n, a.nb, u + du, v + dv, s + ds);
-            const float fm = sample_tsdf<FMT>(a.f, a.n, a.nb, u - du, v - dv, s - ds);
-            const float d = fm - fp;
-            gx = ax == 0 ? d : gx;
-            gy = ax == 1 ? d : gy;
-            gz = ax == 2 ? d : gz;
-        }
+        // the 48 corner loads of the 6 samples are issued together (one round trip)
+        const float gx = sample_tsdf<FMT>(a.f, a.n, a.nb, u - r, v, s) - sample_tsdf<FMT>(a.f, a.n, a.nb, u + r, v, s);
+        const float gy = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v - r, s) - sample_tsdf<FMT>(a.f, a.n, a.nb, u, v + r, s);
+        const float gz = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s - r) - sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s + r);
         const v3 gg = div3s(mk3(gx, gy, gz), r);
         const float gn = len3(gg);
         if (is_depth_valid(gn)) {
