@@ -732,25 +732,33 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
     write_surface_wave<FMT>(a, L, i, j, g, tstar);
 }
 
-// XCD-aware tile order: workgroup g runs on XCD g % 8; give each XCD a
-// contiguous band of tiles.
-__device__ __forceinline__ int xcd_tile(int tiles)
+// Workgroups of the first pass per level: its tile count rounded up to 8.
+__host__ __device__ __forceinline__ int level_blocks(const RaycastArgs& a, int l)
 {
-    const int g = blockIdx.x;
-    const int per_xcd = (tiles + 7) / 8;
-    return (g % 8) * per_xcd + g / 8;
+    const int n = (l + 1 < a.levels ? a.lv[l + 1].tile0 : a.tiles) - a.lv[l].tile0;
+    return (n + 7) & ~7;
 }
 
+// Dispatch order of the first pass: the coarsest level first (a tile of a
+// coarse level covers more of the scene and its waves march longest; started
+// last they made the kernel's tail), each level's tiles in 8 XCD-contiguous
+// bands (workgroup g runs on XCD g % 8), so every XCD gets its share of every
+// level.
 template <int FMT>
 __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.clear = 0u;
-    const int tile = xcd_tile(a.tiles);
-    if (tile >= a.tiles) return;
-    int l = 0;
-    while (l + 1 < a.levels && tile >= a.lv[l + 1].tile0) ++l;
+    int g = blockIdx.x, l = a.levels - 1;
+    for (; l >= 0; --l) {
+        const int n8 = level_blocks(a, l);
+        if (g < n8) break;
+        g -= n8;
+    }
+    if (l < 0) return;
+    const int n8 = level_blocks(a, l);
+    const int lt = (g % 8) * (n8 / 8) + g / 8;
+    if (lt >= (l + 1 < a.levels ? a.lv[l + 1].tile0 : a.tiles) - a.lv[l].tile0) return;
     const RaycastLevel& L = a.lv[l];
-    const int lt = tile - L.tile0;
     const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
     const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
     const RayQueue rq{a.rays, a.count, a.cap};
@@ -976,7 +984,8 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
     a.count = v->ray_count + parity;
     a.clear = v->ray_count + (1 - parity);
     a.cap = (unsigned)v->ray_cap;
-    const unsigned blocks = (unsigned)(((tiles + 7) / 8) * 8);
+    unsigned blocks = 0;
+    for (int l = 0; l < levels; ++l) blocks += (unsigned)level_blocks(a, l);
     if (v->format == KF_VOXEL_F32) {
         raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
         KF_LAUNCHED("raycast_kernel");
