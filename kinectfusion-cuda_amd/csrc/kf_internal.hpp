@@ -89,6 +89,12 @@ struct kf_volume {
     mutable unsigned* ray_count = nullptr;
     mutable size_t ray_cap = 0;
     mutable int64_t ray_launches = 0;
+    // first-pass dispatch order (tiles that took longest last time first):
+    // 2 launch parities x order_cap slots, and its counters and wave times
+    mutable int* tile_order = nullptr;
+    mutable void* order_state = nullptr;
+    mutable size_t order_cap = 0;
+    mutable int64_t order_key = -1;  // (levels, width, height) the stored order was made for
 };
 
 struct kf_icp {

@@ -36,6 +36,20 @@ constexpr float kSkipBound = 0.991f;
 constexpr float kMargin = 0.01f;
 
 
+// Dispatch order of the first pass.  Workgroup g of a level runs on XCD
+// g % 8 and takes the (g / 8)-th tile of its level's band on that XCD; which
+// tile that is comes from the previous launch: there, every workgroup timed
+// its wave and appended its tile to the next launch's order of its band, at
+// the front when the wave took longer than the level's mean, else at the back,
+// so the tiles that marched longest start first (they made the kernel's tail
+// when they started late).  The mean is estimated from one wave in eight,
+// summed per level and XCD (one shared sum per level made every wave wait on
+// the same contended atomic).  Only the order changes, never a result.
+struct TileOrderState {
+    unsigned ctr[2][3][8][2];         // [parity][level][XCD][front, back] appends
+    unsigned long long dur[3][3][8];  // [launch % 3][level][XCD] sum of sampled wave times
+    unsigned waves[3][3][8];          // [launch % 3][level][XCD] waves sampled
+};
 struct RaycastArgs {
     RaycastVolume vol;
     float pose[16];
@@ -46,6 +60,10 @@ struct RaycastArgs {
     unsigned* count;    // their count for this launch
     unsigned* clear;    // the count of the other launch parity, cleared for the next launch
     unsigned cap;
+    const int* order_in;   // band slot of each workgroup, from the previous launch (null: g / 8)
+    int* order_out;        // the same for the next launch (null: not recorded)
+    TileOrderState* ost;
+    int parity, tri;       // launch index mod 2 and mod 3
 };
 
 template <int FMT>
@@ -744,25 +762,67 @@ __host__ __device__ __forceinline__ int level_blocks(const RaycastArgs& a, int l
 // last they made the kernel's tail), each level's tiles in 8 XCD-contiguous
 // bands (workgroup g runs on XCD g % 8), so every XCD gets its share of every
 // level.
+// Appends band slot `slot` of (level l, XCD x) to the next launch's order
+// (lane 0): front when the wave took longer than the level's mean wave time
+// of the previous launch (always, without one), else back.
+__device__ __forceinline__ void record_order(const RaycastArgs& a, int l, int x, int per, int base, int slot,
+                                             unsigned long long dur)
+{
+    TileOrderState* o = a.ost;
+    const int prev = (a.tri + 2) % 3;
+    unsigned long long n = 0, sum = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        n += o->waves[prev][l][q];
+        sum += o->dur[prev][l][q];
+    }
+    const bool front = n == 0 || dur * n > sum;
+    unsigned* c = &o->ctr[1 - a.parity][l][x][front ? 0 : 1];
+    const int k = (int)atomicAdd(c, 1u);
+    if (k < per) a.order_out[base + (front ? k : per - 1 - k) * 8 + x] = slot;
+    if (dur != 0 && ((int)blockIdx.x / 8) % 8 == 0) {  // one workgroup in eight
+        atomicAdd(&o->dur[a.tri][l][x], dur);
+        atomicAdd(&o->waves[a.tri][l][x], 1u);
+    }
+}
+
 template <int FMT>
 __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.clear = 0u;
-    int g = blockIdx.x, l = a.levels - 1;
+    const unsigned long long t0 = wall_clock64();
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *a.clear = 0u;
+        if (a.ost) {
+            // counters the next launch appends with, and the wave times of the launch after it
+            unsigned* c = &a.ost->ctr[a.parity][0][0][0];
+            if (threadIdx.x < 3 * 8 * 2) c[threadIdx.x] = 0u;
+            if (threadIdx.x < 3 * 8) {
+                (&a.ost->dur[(a.tri + 1) % 3][0][0])[threadIdx.x] = 0ull;
+                (&a.ost->waves[(a.tri + 1) % 3][0][0])[threadIdx.x] = 0u;
+            }
+        }
+    }
+    int g = blockIdx.x, l = a.levels - 1, base = 0;
     for (; l >= 0; --l) {
         const int n8 = level_blocks(a, l);
         if (g < n8) break;
         g -= n8;
+        base += n8;
     }
     if (l < 0) return;
-    const int n8 = level_blocks(a, l);
-    const int lt = (g % 8) * (n8 / 8) + g / 8;
-    if (lt >= (l + 1 < a.levels ? a.lv[l + 1].tile0 : a.tiles) - a.lv[l].tile0) return;
-    const RaycastLevel& L = a.lv[l];
-    const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
-    const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-    const RayQueue rq{a.rays, a.count, a.cap};
-    raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h, rq, l);
+    const int n8 = level_blocks(a, l), per = n8 / 8, x = g % 8;
+    const int slot = a.order_in ? a.order_in[blockIdx.x] : g / 8;
+    const int lt = x * per + slot;
+    const bool live = slot >= 0 && slot < per && lt < (l + 1 < a.levels ? a.lv[l + 1].tile0 : a.tiles) - a.lv[l].tile0;
+    if (live) {
+        const RaycastLevel& L = a.lv[l];
+        const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
+        const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
+        const RayQueue rq{a.rays, a.count, a.cap};
+        raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h, rq, l);
+    }
+    if (a.order_out && threadIdx.x == 0)
+        record_order(a, l, x, per, base, slot, live ? wall_clock64() - t0 : 0ull);
 }
 
 // Workgroups (one wave each) of the finishing pass; each takes every
@@ -953,6 +1013,24 @@ static int ensure_ray_list(const kf_volume* v, size_t rays, hipStream_t s)
     return KF_OK;
 }
 
+// The first pass's dispatch-order state of a volume, for `blocks` workgroups.
+static int ensure_tile_order(const kf_volume* v, size_t blocks, hipStream_t s)
+{
+    if (!v->order_state) {
+        KF_HIP(hipMalloc(&v->order_state, sizeof(TileOrderState)));
+        KF_HIP(hipMemset(v->order_state, 0, sizeof(TileOrderState)));
+    }
+    if (v->order_cap >= blocks) return KF_OK;
+    KF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(v->tile_order);
+    v->tile_order = nullptr;
+    v->order_cap = 0;
+    v->order_key = -1;
+    KF_HIP(hipMalloc(&v->tile_order, 2 * sizeof(int) * blocks));
+    v->order_cap = blocks;
+    return KF_OK;
+}
+
 int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState* dyn_state, int levels,
                           const float (*inv_k)[9], float4* const* vout, float4* const* nout, int w0, int h0,
                           hipStream_t s)
@@ -979,13 +1057,24 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
         tiles += L.tiles_x * ((L.h + 7) / 8);
     }
     a.tiles = tiles;
-    const int parity = (int)(v->ray_launches++ & 1);
+    unsigned blocks = 0;
+    for (int l = 0; l < levels; ++l) blocks += (unsigned)level_blocks(a, l);
+    st = ensure_tile_order(v, blocks, s);
+    if (st) return st;
+    const int64_t launch = v->ray_launches++;
+    const int parity = (int)(launch & 1);
     a.rays = static_cast<DeferredRay*>(v->ray_list);
     a.count = v->ray_count + parity;
     a.clear = v->ray_count + (1 - parity);
     a.cap = (unsigned)v->ray_cap;
-    unsigned blocks = 0;
-    for (int l = 0; l < levels; ++l) blocks += (unsigned)level_blocks(a, l);
+    // the order recorded by the previous launch is used when it was made for the same maps
+    const int64_t key = ((int64_t)levels << 42) | ((int64_t)w0 << 21) | (int64_t)h0;
+    a.order_in = v->order_key == key ? v->tile_order + (size_t)parity * v->order_cap : nullptr;
+    a.order_out = v->tile_order + (size_t)(1 - parity) * v->order_cap;
+    a.ost = static_cast<TileOrderState*>(v->order_state);
+    a.parity = parity;
+    a.tri = (int)(launch % 3);
+    v->order_key = key;
     if (v->format == KF_VOXEL_F32) {
         raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
         KF_LAUNCHED("raycast_kernel");

@@ -832,6 +832,8 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->dmax);
     (void)hipFree(v->ray_list);
     (void)hipFree(v->ray_count);
+    (void)hipFree(v->tile_order);
+    (void)hipFree(v->order_state);
     delete v;
     return KF_OK;
 }

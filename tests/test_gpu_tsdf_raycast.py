@@ -55,19 +55,22 @@ def test_raycast_bit_exact(kf, oracle32, L):
     n, mue, w, h = 128, 0.03, 320, 240
     seq, grid, sensor, F, W, Fg, Wg, _, _ = fuse_case(kf, oracle32, 9, n, L, mue, w, h, 3)
     for lvl in range(3):
-        announce(f"raycast n {n} L {L} level {lvl}")
         lw, lh = w >> lvl, h >> lvl
         vm, nm = kf.GridMap(lw, lh, 4), kf.GridMap(lw, lh, 4)
         init = np.zeros((lh, lw, 4), np.float32)
         init[..., 3] = -2
-        vm.upload(init)
-        nm.upload(init)
-        kf.kernel.raycast(grid, sensor, vm, nm, lvl)
         rv, rn = oracle32.raycast(F, n, L, mue, sensor.get_pose(), sensor.get_inverse_intr(lvl), lw, lh)
-        gv, gn = vm.download(), nm.download()
         assert (rv[..., 3] == -1).mean() > 0.8
-        np.testing.assert_array_equal(gv.view(np.uint32), rv.view(np.uint32))
-        np.testing.assert_array_equal(gn.view(np.uint32), rn.view(np.uint32))
+        # level 0 is cast three times in a row: the repeats dispatch their tiles in
+        # the order the previous launch recorded, which must not change the maps
+        for rep in range(3 if lvl == 0 else 1):
+            announce(f"raycast n {n} L {L} level {lvl} launch {rep}")
+            vm.upload(init)
+            nm.upload(init)
+            kf.kernel.raycast(grid, sensor, vm, nm, lvl)
+            gv, gn = vm.download(), nm.download()
+            np.testing.assert_array_equal(gv.view(np.uint32), rv.view(np.uint32))
+            np.testing.assert_array_equal(gn.view(np.uint32), rn.view(np.uint32))
 
 
 def test_raycast_stale_pixels(kf, oracle32):
