@@ -124,6 +124,29 @@ __device__ __forceinline__ float kf_expf(float x)
     if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
     return y * __int_as_float((k + 127) << 23);
 }
+// kf_expf for x <= 0 or NaN (the bilateral weights, -a - b with a, b >= 0):
+// the overflow branches of kf_expf are never taken there, so the same
+// operations without them give the same bits.
+__device__ __forceinline__ float kf_expf_nonpos(float x)
+{
+    if (!(x == x)) return x;
+    if (x < -104.0f) return 0.0f;
+    float fk = floorf(x * 1.44269504088896341f + 0.5f);
+    int k = (int)fk;
+    float r = x - fk * 0.693359375f;
+    r = r - fk * -2.12194440e-4f;
+    float z = r * r;
+    float p = 1.9875691500e-4f;
+    p = p * r + 1.3981999507e-3f;
+    p = p * r + 8.3334519073e-3f;
+    p = p * r + 4.1665795894e-2f;
+    p = p * r + 1.6666665459e-1f;
+    p = p * r + 5.0000001201e-1f;
+    float y = p * z + r;
+    y = y + 1.0f;
+    if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
+    return y * __int_as_float((k + 127) << 23);
+}
 
 // Live normal at (i,j) of a vertex map (device_helper.cuh:81-110).
 // The three vertices a normal needs: the pixel, its right and its lower

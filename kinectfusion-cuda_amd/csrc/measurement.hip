@@ -47,7 +47,7 @@ __device__ __forceinline__ float bilateral_at(const BilateralTile& tile, int tx,
             float sd = (float)(x * x + y * y);
             float id = center - cur;
             id = id * id;
-            float f = kf_expf(-sd * inv_s - id * inv_r);
+            float f = kf_expf_nonpos(-sd * inv_s - id * inv_r);
             norm = norm + f;
             acc = acc + f * cur;
         }
