@@ -101,7 +101,8 @@ __device__ __forceinline__ float half_rne(uint32_t c)
 }
 
 // Single-precision exp: Cody-Waite reduction by ln2 with the Cephes expf
-// polynomial (S. L. Moshier), then exact scaling by 2^k with one final rounding.
+// polynomial (S. L. Moshier) in fused multiply-adds (the oracle's kfo_expf
+// makes the same fmaf calls), then exact scaling by 2^k with one final rounding.
 __device__ __forceinline__ float kf_expf(float x)
 {
     if (!(x == x)) return x;
@@ -109,16 +110,16 @@ __device__ __forceinline__ float kf_expf(float x)
     if (x > 88.72283f) return __int_as_float(0x7f800000);
     float fk = floorf(x * 1.44269504088896341f + 0.5f);
     int k = (int)fk;
-    float r = x - fk * 0.693359375f;
-    r = r - fk * -2.12194440e-4f;
+    float r = x - fk * 0.693359375f;  // exact product (9-bit constant)
+    r = fmaf(fk, 2.12194440e-4f, r);
     float z = r * r;
     float p = 1.9875691500e-4f;
-    p = p * r + 1.3981999507e-3f;
-    p = p * r + 8.3334519073e-3f;
-    p = p * r + 4.1665795894e-2f;
-    p = p * r + 1.6666665459e-1f;
-    p = p * r + 5.0000001201e-1f;
-    float y = p * z + r;
+    p = fmaf(p, r, 1.3981999507e-3f);
+    p = fmaf(p, r, 8.3334519073e-3f);
+    p = fmaf(p, r, 4.1665795894e-2f);
+    p = fmaf(p, r, 1.6666665459e-1f);
+    p = fmaf(p, r, 5.0000001201e-1f);
+    float y = fmaf(p, z, r);
     y = y + 1.0f;
     if (k > 127) { y = y * 1.7014118346046923e38f; k -= 127; }
     if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
@@ -133,16 +134,16 @@ __device__ __forceinline__ float kf_expf_nonpos(float x)
     if (x < -104.0f) return 0.0f;
     float fk = floorf(x * 1.44269504088896341f + 0.5f);
     int k = (int)fk;
-    float r = x - fk * 0.693359375f;
-    r = r - fk * -2.12194440e-4f;
+    float r = x - fk * 0.693359375f;  // exact product (9-bit constant)
+    r = fmaf(fk, 2.12194440e-4f, r);
     float z = r * r;
     float p = 1.9875691500e-4f;
-    p = p * r + 1.3981999507e-3f;
-    p = p * r + 8.3334519073e-3f;
-    p = p * r + 4.1665795894e-2f;
-    p = p * r + 1.6666665459e-1f;
-    p = p * r + 5.0000001201e-1f;
-    float y = p * z + r;
+    p = fmaf(p, r, 1.3981999507e-3f);
+    p = fmaf(p, r, 8.3334519073e-3f);
+    p = fmaf(p, r, 4.1665795894e-2f);
+    p = fmaf(p, r, 1.6666665459e-1f);
+    p = fmaf(p, r, 5.0000001201e-1f);
+    float y = fmaf(p, z, r);
     y = y + 1.0f;
     if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
     return y * __int_as_float((k + 127) << 23);

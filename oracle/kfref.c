@@ -145,8 +145,9 @@ float KFO(u16_to_half_float)(uint16_t code) { return half_rne(code); }
  * built with --use_fast_math (project.vcxproj:139); its result is not
  * reproducible here.  Both this oracle and the HIP kernel use the same
  * project-written single-precision exp: Cody-Waite reduction by ln2 and the
- * degree-6 polynomial of the Cephes library's expf (S. L. Moshier), followed by
- * an exact power-of-two scaling with a single final rounding (denormals kept).
+ * degree-6 polynomial of the Cephes library's expf (S. L. Moshier), evaluated
+ * with fused multiply-adds (fmaf: one rounding per step, on both sides), followed
+ * by an exact power-of-two scaling with a single final rounding (denormals kept).
  * Pinned by: max relative error vs FP64 exp over a dense sweep of [-104, 88]
  * (tests/test_oracle_measurement.py). */
 #ifndef KFREF_F64
@@ -157,16 +158,16 @@ static float kf_expf_impl(float x)
     if (x > 88.72283f) return INFINITY;
     float fk = floorf(x * 1.44269504088896341f + 0.5f);
     int k = (int)fk;
-    float r = x - fk * 0.693359375f;
-    r = r - fk * -2.12194440e-4f;
+    float r = x - fk * 0.693359375f;  /* exact product (9-bit constant) */
+    r = fmaf(fk, 2.12194440e-4f, r);
     float z = r * r;
     float p = 1.9875691500e-4f;
-    p = p * r + 1.3981999507e-3f;
-    p = p * r + 8.3334519073e-3f;
-    p = p * r + 4.1665795894e-2f;
-    p = p * r + 1.6666665459e-1f;
-    p = p * r + 5.0000001201e-1f;
-    float y = p * z + r;
+    p = fmaf(p, r, 1.3981999507e-3f);
+    p = fmaf(p, r, 8.3334519073e-3f);
+    p = fmaf(p, r, 4.1665795894e-2f);
+    p = fmaf(p, r, 1.6666665459e-1f);
+    p = fmaf(p, r, 5.0000001201e-1f);
+    float y = fmaf(p, z, r);
     y = y + 1.0f;
     if (k > 127) { y = y * 1.7014118346046923e38f; k -= 127; }     /* * 2^127, exact */
     if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }      /* * 2^-64, exact */
