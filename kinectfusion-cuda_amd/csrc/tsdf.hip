@@ -166,8 +166,13 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         const float d = dep[kl];
         const float diff = d - pz[kl];
         if (!(hit[kl] && is_depth_valid(d) && diff >= -a.mue)) continue;
-        float f = diff / a.mue;
-        f = (f < 1.0f) ? f : 1.0f;
+        // f = min(1, diff / mue): diff >= mue makes the rounded quotient >= 1, so the
+        // division (most updated voxels lie in front of the surface) is skipped there
+        float f = 1.0f;
+        if (diff < a.mue) {
+            f = diff / a.mue;
+            f = (f < 1.0f) ? f : 1.0f;
+        }
         const int64_t q = base + 64 * kl;
         if (FMT == KF_VOXEL_F32) {
             const float wt = 1.0f / d;
