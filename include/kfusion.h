@@ -165,7 +165,8 @@ int kf_solve_normal_equations(const float* mat_a, const float* vec_b, unsigned i
 /* One ICP iteration's association fused with its reduction: the 27 sums of the
  * rows kf_construct_icp_residuals would produce, plus the number of accepted
  * rows, without materialising A and b.  sums_host: 21 upper-triangle entries of
- * A^T A in row order (p <= q), then A^T b; as doubles. */
+ * A^T A in row order (p <= q), then A^T b; as doubles.  w x h <= 2^24
+ * (KF_ERR_DIMS above). */
 int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const float* target_normal_map,
                   int w, int h, const float prev_rot[9], const float prev_transl[3], const float cur_rot[9],
                   const float cur_transl[3], const float intrinsics[9], float distance_thresh, float angle_thresh,
@@ -175,7 +176,7 @@ int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const
 typedef struct kf_icp_config {
     int iters_per_layer[3];              /* index 0 = finest level */
     int levels;                          /* 1..3 */
-    int width, height;                   /* level-0 map size */
+    int width, height;                   /* level-0 map size; width x height <= 2^24 */
     float distance_thresh;               /* metres */
     float angle_thresh;                  /* radians */
     float iteration_stop_thresh_angle;   /* poseError chord (general_helper.cpp:11-21); 0 disables */
@@ -286,7 +287,7 @@ int kf_pose_error(const float pose1[16], const float pose2[16], float* angle, fl
  * order), whatever else shares its ticks.  No reference counterpart: this
  * is the multi-session form of kf_pipeline_frame. */
 typedef struct kf_session_config {
-    int frame_width, frame_height;  /* divisible by 4 (3-level pyramid) */
+    int frame_width, frame_height;  /* divisible by 4 (3-level pyramid); width x height <= 2^24 */
     float depth_scale;
     float fx, fy, cx, cy;
     int volume_n;                   /* multiple of 8 */
