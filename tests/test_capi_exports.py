@@ -72,3 +72,25 @@ def test_oracle_and_library_intrinsics_agree():
         K, Ki = o.intrinsics(517.3, 516.5, 318.6, 255.3, lvl)
         np.testing.assert_array_equal(np.array(s.intr[lvl][:], np.float32), K)
         np.testing.assert_array_equal(np.array(s.inv_intr[lvl][:], np.float32), Ki)
+
+
+def test_icp_map_size_limit_without_device():
+    """ICP maps above 2^24 pixels are refused before any device call (the kernels
+    take pixel coordinates from an FP32 quotient that is exact only below it)."""
+    from kfusion import _capi
+    lib = _capi.load()
+    cfg = _capi.kf_icp_config()
+    cfg.iters_per_layer[0] = 1
+    cfg.levels = 1
+    cfg.width, cfg.height = 8192, 4097  # one row more than 2^24 pixels
+    out = C.c_void_p()
+    assert lib.kf_icp_create(C.byref(cfg), C.byref(out)) == _capi.KF_ERR_DIMS
+    assert b"2^24" in lib.kf_last_error()
+    buf = C.c_void_p(1)  # never dereferenced: the size check comes first
+    sums = (C.c_double * 27)()
+    n = C.c_int64()
+    m9 = (C.c_float * 9)()
+    m3 = (C.c_float * 3)()
+    rc = lib.kf_icp_reduce(buf, buf, buf, 8192, 4097, m9, m3, m9, m3, m9, C.c_float(0.1), C.c_float(0.3), 0, sums,
+                           C.byref(n), None, None)
+    assert rc == _capi.KF_ERR_DIMS
