@@ -282,7 +282,10 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
 // takes hundreds of small steps) is appended to a list that a second pass
 // finishes with a whole wave per ray, so no wave of the first pass waits on
 // one long ray.
-constexpr int kLaneBudget = 48;
+// The single-session first pass (its waves' tail bounds the frame) hands rays
+// over after 40 samples; the batch's persistent pass (throughput-bound) after 48.
+constexpr int kLaneBudget = 40;
+constexpr int kBatchLaneBudget = 48;
 constexpr int kWorkFloor = 16;
 
 // One trilinear evaluation of the march at t: 1 = skipped (the sample is known
@@ -450,6 +453,7 @@ struct RayQueue {
     DeferredRay* rays;
     unsigned* count;
     unsigned cap;
+    int budget;  // samples a lane marches before it may hand its ray over
 };
 
 // raycast.cu:44-145 for pixel (i, j) of one level map, one lane per pixel,
@@ -479,7 +483,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     for (;;) {
         if (!(t < farv)) break;
         if (++guard > kMaxMarchSteps) break;
-        if (can_defer && guard > kLaneBudget && work > kWorkFloor) {
+        if (can_defer && guard > rq.budget && work > kWorkFloor) {
             const unsigned slot = atomicAdd(rq.count, 1u);
             if (slot < rq.cap) {
                 DeferredRay d;
@@ -818,7 +822,7 @@ __global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
         const RaycastLevel& L = a.lv[l];
         const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
         const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-        const RayQueue rq{a.rays, a.count, a.cap};
+        const RayQueue rq{a.rays, a.count, a.cap, kLaneBudget};
         raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h, rq, l);
     }
     if (a.order_out && threadIdx.x == 0)
@@ -888,7 +892,7 @@ template <int FMT>
 __global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd, const RaycastJob* jobs)
 {
     const RayQueue rq{hd->ray_list + (FMT == KF_VOXEL_F32 ? 0u : hd->ray_list_cap),
-                      hd->ray_next + (FMT == KF_VOXEL_F32 ? 16 : 17), hd->ray_list_cap};
+                      hd->ray_next + (FMT == KF_VOXEL_F32 ? 16 : 17), hd->ray_list_cap, kBatchLaneBudget};
     const int nf = hd->n_ray_f32, tf = hd->ray_tiles_f32;
     const int j0 = FMT == KF_VOXEL_F32 ? 0 : nf, j1 = FMT == KF_VOXEL_F32 ? nf : hd->n_ray;
     const int t0 = FMT == KF_VOXEL_F32 ? 0 : tf, t1 = FMT == KF_VOXEL_F32 ? tf : hd->ray_tiles;
