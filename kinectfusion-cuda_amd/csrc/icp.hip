@@ -681,7 +681,8 @@ __device__ void pose_error(const float* R1, const float* t1, const float* R2, co
 
 // Sums the partial rows (fixed order) into 27 doubles + count in LDS.  Wave w
 // owns rows q = w, w + 4, ... (7 of the 28); lane l adds blocks l, l + 64, ...
-// in that order, then the lanes combine by butterfly.  The loads of all of a
+// in that order, then the lanes combine (wave_reduce / wave_comp_sum: lane 0
+// ends with the total, the lower lane's operand first).  The loads of all of a
 // wave's rows for a chunk of kRowHalves 64-block groups (RowChunk) are issued
 // before any is added, so a row set costs one memory round trip per 320 blocks
 // (a 640x480 level is 300); the first chunk can be loaded by the caller ahead

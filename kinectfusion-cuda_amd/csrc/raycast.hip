@@ -378,9 +378,9 @@ __device__ __forceinline__ RayGeom ray_geom(const RaycastVolume& a, const float*
 }
 
 // raycast.cu:29-42 and 119-145: the surface point at tstar (tstar == 0: no
-// crossing) and its normal -grad F, written by one lane.  The gradient takes
-// its 6 samples one axis at a time (fewer live registers); x + 0 == x, so the
-// samples are those of raycast.cu:33-38.
+// crossing) and its normal -grad F, written by one lane.  The 6 gradient
+// samples' corner loads are issued together; x + 0 == x, so the samples are
+// those of raycast.cu:33-38.
 template <int FMT>
 __device__ __forceinline__ void write_surface(const RaycastVolume& a, const RaycastLevel& L, int i, int j,
                                               const RayGeom& g, float tstar)
