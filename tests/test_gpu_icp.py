@@ -118,6 +118,33 @@ def test_residual_rows_bit_exact(kf, oracle32):
             np.testing.assert_allclose(sums, ref, rtol=0, atol=tol, err_msg=f"compensated={comp}")
 
 
+def test_residual_rows_threshold_sweep(kf, oracle32):
+    """The distance and angle tests are decided from cheap forms except inside a
+    narrow band around the threshold, where they are evaluated exactly.  Dense
+    sweeps of both thresholds over the range the pixels span put pixels inside
+    those bands; every row must still equal the oracle's bit for bit."""
+    seq, Ks, P, vp, tv, tn = model_maps(kf, oracle32)
+    R, t = rt(P)
+    Rc = R.copy()
+    tc = t + np.array([0.004, -0.003, 0.002], np.float32)
+    h, w = vp[0].shape[:2]
+    V, TV, TN = upload_pyr(kf, [vp[0], tv[0], tn[0]])
+    A = kf.DeviceArray((h * w, 6))
+    b = kf.DeviceArray((h * w,))
+    sweeps = [(float(d), np.deg2rad(20.0)) for d in np.geomspace(0.002, 0.1, 24)]
+    sweeps += [(0.1, float(np.deg2rad(a))) for a in np.linspace(1.0, 45.0, 24)]
+    accepted = []
+    for dist, ang in sweeps:
+        announce(f"constructIcpResiduals threshold sweep distance {dist:.4f} m angle {np.rad2deg(ang):.2f} deg")
+        kf.kernel.construct_icp_residuals(V, TV, TN, R, t, Rc, tc, Ks[0][0], dist, ang, A, b)
+        Ar, br = oracle32.construct_icp_residuals(vp[0], tv[0], tn[0], R, t, Rc, tc, Ks[0][0], dist, ang)
+        np.testing.assert_array_equal(A.download().view(np.uint32), Ar.view(np.uint32))
+        np.testing.assert_array_equal(b.download().view(np.uint32), br.view(np.uint32))
+        accepted.append(int(np.count_nonzero(np.any(Ar != 0, axis=1))))
+    # the sweeps move the decisions: the accepted counts grow with both thresholds
+    assert accepted[0] < accepted[23] and accepted[24] < accepted[-1]
+
+
 def test_compute_pose_matches_oracle(kf, oracle32):
     seq, Ks, P, vp, tv, tn = model_maps(kf, oracle32, seed=13)
     R, t = rt(P)
