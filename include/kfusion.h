@@ -137,7 +137,9 @@ int kf_fuse(const float* depth_m, int w, int h, kf_volume* v, const kf_sensor* s
             float* elapsed_ms);
 /* kernel::raycast (raycast.cuh:11, raycast.cu:149-169): world-space vertex and
  * normal maps of level `level` seen from sensor.pose; pixels whose ray misses
- * the volume or the [0.4, 6] m range are left untouched (raycast.cu:61-73). */
+ * the volume or the [0.4, 6] m range are left untouched (raycast.cu:61-73).
+ * The volume keeps per-launch scratch (handed-over rays, tile order): issue a
+ * volume's raycasts and fusions in order on one stream. */
 int kf_raycast(const kf_volume* v, const kf_sensor* sensor, float* vertex_map, float* normal_map, int w, int h,
                int level, kf_stream_t stream, float* elapsed_ms);
 /* VoxelGrid::saveVoxelGrid / loadVoxelGrid (voxel_grid.cpp:43-80): raw
