@@ -66,12 +66,6 @@ struct RaycastArgs {
     int parity, tri;       // launch index mod 2 and mod 3
 };
 
-template <int FMT>
-__device__ __forceinline__ float voxel_f(const void* f, int64_t q)
-{
-    if (FMT == KF_VOXEL_F32) return static_cast<const float*>(f)[q];
-    return snorm_decode(static_cast<const int16_t*>(f)[q]);
-}
 
 // Normalized texture coordinate x / width + 1/2 of a world coordinate.  For a
 // power-of-two width the quotient is the product with the exact reciprocal
