@@ -39,7 +39,9 @@ __device__ __forceinline__ float bilateral_at(const BilateralTile& tile, int tx,
     const float inv_r = 1.0f / (0.25f * 0.25f);
     float center = tile[ty + 3][tx + 3];
     float norm = 0.0f, acc = 0.0f;
-#pragma unroll
+    // the outer loop stays rolled: unrolled, all 49 taps' tile reads were held
+    // at once (106 VGPRs, 4 waves per SIMD); rolled, 27 VGPRs and 8 waves
+#pragma unroll 1
     for (int x = -3; x <= 3; ++x) {
 #pragma unroll
         for (int y = -3; y <= 3; ++y) {
