@@ -784,8 +784,10 @@ __device__ __forceinline__ void record_order(const RaycastArgs& a, int l, int x,
     }
 }
 
+// Five waves per SIMD: 96 VGPRs (from 105) and 20 bytes of spill per lane for the FP32 form;
+// measured 1 % faster than four. The batch form stays at four (five made it 2 % slower).
 template <int FMT>
-__global__ __launch_bounds__(64) void raycast_kernel(RaycastArgs a)
+__global__ __launch_bounds__(64, 5) void raycast_kernel(RaycastArgs a)
 {
     const unsigned long long t0 = wall_clock64();
     if (blockIdx.x == 0) {
