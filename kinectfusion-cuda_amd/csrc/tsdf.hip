@@ -419,8 +419,10 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
 // Each wave sweeps a contiguous run of the list: consecutive entries mostly
 // belong to one session (the cull appends runs of up to 256 bricks of a job),
 // so the job's arguments and pose are reloaded only when the job changes, and
-// the next entry is fetched while a brick is fused.
-__global__ __launch_bounds__(64 * kFuseWaves) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
+// the next entry is fetched while a brick is fused. Eight waves per SIMD: the
+// job arguments otherwise hold 106 SGPRs (seven waves); at eight, 40 of them
+// move to VGPR lanes, no scratch, and a 64-session tick measured 0.6 % faster.
+__global__ __launch_bounds__(64 * kFuseWaves, 8) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
 {
     __shared__ int wave_cnt[kFuseWaves];
     // the tick's table, the list and its length are never written while the
