@@ -313,7 +313,8 @@ typedef struct kf_batch_config {
 typedef struct kf_batch kf_batch;
 int kf_batch_create(const kf_batch_config* config, kf_stream_t stream, kf_batch** out);
 int kf_batch_destroy(kf_batch* b);
-/* HBM a session holds while admitted (volume + maps + ICP state), in bytes. */
+/* HBM a session holds while admitted, in bytes: volume, maps and ICP state, plus the most
+ * it can add to the batch's shared fuse and deferred-ray lists (which grow 2x at a time). */
 int kf_session_bytes(const kf_session_config* config, size_t* bytes);
 int kf_batch_add_session(kf_batch* b, const kf_session_config* config, int* session);
 int kf_batch_remove_session(kf_batch* b, int session);
@@ -350,7 +351,8 @@ int kf_batch_session_restore(kf_batch* b, int session, const char* path);
  * No reference counterpart (the reference serves one camera per process). */
 typedef struct kf_server_config {
     kf_batch_config batch;       /* max_sessions = resident session slots */
-    size_t hbm_budget_bytes;     /* admission budget over kf_session_bytes of the resident sessions */
+    size_t hbm_budget_bytes;     /* admission budget: each resident session is charged kf_session_bytes
+                                    plus its device frame buffer (2 bytes per pixel) */
     int queue_depth;             /* frames buffered per session (pinned host memory); 0 = 8 */
 } kf_server_config;
 typedef struct kf_server_stats {

@@ -151,6 +151,17 @@ size_t maps_bytes(int w, int h)
            sizeof(kf::PoseState);
 }
 
+}  // namespace
+
+size_t kf::batch_shared_bytes(const kf_session_config* c)
+{
+    const size_t nb = (size_t)c->volume_n / 8, w = (size_t)c->frame_width, h = (size_t)c->frame_height;
+    const size_t rays = w * h + (w / 2) * (h / 2) + (w / 4) * (h / 4);
+    return 2 * (sizeof(unsigned long long) * nb * nb * nb + 2 * sizeof(kf::DeferredRay) * rays);
+}
+
+namespace {
+
 int check_session_config(const kf_session_config* c)
 {
     if (c->frame_width <= 0 || c->frame_height <= 0 || c->frame_width % 4 || c->frame_height % 4)
@@ -370,7 +381,8 @@ int kf_session_bytes(const kf_session_config* c, size_t* bytes)
     if (st) return st;
     const size_t n = (size_t)c->volume_n, vox = n * n * n;
     const size_t planes = vox * (c->voxel_format == KF_VOXEL_F32 ? 8 : 3);
-    *bytes = planes + volume_meta_bytes(c->volume_n) + maps_bytes(c->frame_width, c->frame_height);
+    *bytes = planes + volume_meta_bytes(c->volume_n) + maps_bytes(c->frame_width, c->frame_height) +
+             kf::batch_shared_bytes(c);
     return KF_OK;
 }
 

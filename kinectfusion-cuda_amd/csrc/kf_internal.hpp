@@ -13,6 +13,10 @@ namespace kf {
 
 // Records a detail message for kf_last_error() and returns `status`.
 int fail(int status, const std::string& msg);
+// Device bytes a session can add to a batch's shared lists (its bricks in the
+// fuse list, its rays in the deferred-ray list of each voxel format), counted
+// twice because both lists grow geometrically (batch.hip).
+size_t batch_shared_bytes(const kf_session_config* c);
 int fail_hip(hipError_t e, const char* what);
 void clear_error();
 

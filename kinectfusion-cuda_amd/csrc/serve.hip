@@ -107,10 +107,19 @@ int admit(kf_server* s)
         int free_slots = 0;
         for (const Ticket& o : s->t) free_slots += (o.state == 1) ? 1 : 0;
         if (free_slots >= s->cfg.batch.max_sessions) return KF_OK;
-        int st = kf_batch_add_session(s->b, &q.cfg, &q.sid);
-        if (st) return st;
+        // the frame buffer first: a failure then leaves nothing half admitted
         hipError_t e = hipMalloc(&q.raw_dev, q.frame_bytes);
-        if (e != hipSuccess) return kf::fail_hip(e, "server session buffer");
+        if (e != hipSuccess) {
+            q.raw_dev = nullptr;
+            return kf::fail_hip(e, "server session buffer");
+        }
+        int st = kf_batch_add_session(s->b, &q.cfg, &q.sid);
+        if (st) {
+            (void)hipFree(q.raw_dev);
+            q.raw_dev = nullptr;
+            q.sid = -1;
+            return st;
+        }
         q.state = 1;
         s->resident += q.bytes;
         s->peak = std::max(s->peak, s->resident);
@@ -242,9 +251,12 @@ int kf_server_open(kf_server* s, const kf_session_config* c, int* ticket)
     q.cfg = *c;
     int st = kf_session_bytes(c, &q.bytes);
     if (st) return st;
+    // charged against the budget: what the batch holds for the session and the
+    // session's device frame buffer
+    q.frame_bytes = sizeof(uint16_t) * (size_t)c->frame_width * c->frame_height;
+    q.bytes += q.frame_bytes;
     if (q.bytes > s->cfg.hbm_budget_bytes)
         return kf::fail(KF_ERR_BUDGET, "kf_server_open: the session alone exceeds the HBM budget");
-    q.frame_bytes = sizeof(uint16_t) * (size_t)c->frame_width * c->frame_height;
     q.cap = s->cfg.queue_depth;
     q.arrival.assign((size_t)q.cap, 0.0);
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&q.ring), q.frame_bytes * (size_t)q.cap,

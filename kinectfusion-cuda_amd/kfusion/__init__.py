@@ -10,6 +10,7 @@ wrappers.  Every call goes to HIP kernels; there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -98,6 +99,8 @@ class DeviceArray:
             self.ptr = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.free()
         except Exception:
@@ -342,6 +345,8 @@ class VoxelGrid:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:
@@ -406,6 +411,8 @@ class ICP:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:
@@ -506,6 +513,8 @@ class Pipeline:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:
@@ -630,6 +639,8 @@ class Batch:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:
@@ -753,6 +764,8 @@ class Server:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:
@@ -787,6 +800,8 @@ class TumSequence:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():  # the HIP runtime may already be torn down; the process frees everything
+            return
         try:
             self.close()
         except Exception:

@@ -358,7 +358,11 @@ void KFO(initialize_grid)(real* F, real* W, int n, real f0, real w0)
  * unobserved value 2 of tsdf.cu:7-18 (any f > 1 is stored as it), so trilinear
  * samples next to unobserved space mix in 2 exactly as the FP32 volume does.
  * W counts observations: F = (F*W + f) / (W + 1), W = min(W + 1, 128).  Same
- * projection and tests as the FP32 form. */
+ * projection and tests as the FP32 form.
+ * Count-weight form (format 2, both builds): the precision form's update rule on
+ * unquantised F and W planes of `real` -- compiled as FP64 it is the error
+ * reference of the precision form, which then differs from it only by the
+ * snorm16 rounding of F and FP32 arithmetic. */
 #define KF_SNORM R(32767.0)
 #define KF_SNORM_UNOBSERVED (-32768)
 static inline real snorm_decode(int16_t q)
@@ -406,7 +410,13 @@ static int64_t fuse_impl(void* Fp, void* Wp, int fmt, int n, real width, real mu
                     real f = diff / mue;
                     f = (f < R(1.0)) ? f : R(1.0);   /* glm::min(1, f) = (f < 1) ? f : 1 */
                     int64_t q = ((int64_t)k * n + j) * n + i;
-                    if (fmt == 0) {
+                    if (fmt == 2) {
+                        real* F = (real*)Fp;
+                        real* W = (real*)Wp;
+                        real Wv = W[q];
+                        F[q] = (F[q] * Wv + f) / (Wv + R(1.0));
+                        W[q] = Wv < R(128.0) ? Wv + R(1.0) : R(128.0);
+                    } else if (fmt == 0) {
                         real* F = (real*)Fp;
                         real* W = (real*)Wp;
                         real wt = R(1.0) / d;
@@ -465,7 +475,7 @@ static inline real lerp_(real p, real q, real t) { return (R(1.0) - t) * p + t *
 static inline int64_t vidx(int n, int i, int j, int k) { return ((int64_t)k * n + j) * n + i; }
 static inline real voxel_value(const void* F, int fmt, int64_t q)
 {
-    return fmt == 0 ? ((const real*)F)[q] : snorm_decode(((const int16_t*)F)[q]);
+    return fmt != 1 ? ((const real*)F)[q] : snorm_decode(((const int16_t*)F)[q]);
 }
 static real sample_any(const void* F, int fmt, int n, real u, real v, real s)
 {
@@ -953,7 +963,7 @@ struct KFO(pipeline) {
     int iters[3];
     real K[3][9], Ki[3][9];
     real pose[16], inv_pose[16];
-    int fmt;          /* 0: FP32 F/W, 1: snorm16 F + uint8 W (FP32 build only) */
+    int fmt;          /* 0: 1/d weights, 1: snorm16 F + uint8 W (FP32 build only), 2: count weights */
     int16_t* Fq;
     uint8_t* Wq;
     real *F, *W, *raw_m, *dpyr[3];
@@ -1036,7 +1046,7 @@ int KFO(pipeline_frame)(pipe_t* p, const uint16_t* raw)
         double t1 = now_ms();
         p->stage_ms[0] += t1 - t0;
         for (int l = 0; l < p->levels; ++l)
-            raycast_impl(p->fmt ? (const void*)p->Fq : (const void*)p->F, p->fmt, p->n, p->width, p->mue, p->pose,
+            raycast_impl(p->fmt == 1 ? (const void*)p->Fq : (const void*)p->F, p->fmt, p->n, p->width, p->mue, p->pose,
                          p->Ki[l], w >> l, h >> l, p->pv[l], p->pn[l]);
         double t2 = now_ms();
         p->stage_ms[1] += t2 - t1;
@@ -1059,8 +1069,9 @@ int KFO(pipeline_frame)(pipe_t* p, const uint16_t* raw)
         t0 = now_ms();
     }
     if (status == 0)
-        p->touched = p->fmt ? fuse_impl(p->Fq, p->Wq, 1, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0])
-                            : fuse_impl(p->F, p->W, 0, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0]);
+        p->touched = p->fmt == 1 ? fuse_impl(p->Fq, p->Wq, 1, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose, p->K[0])
+                                 : fuse_impl(p->F, p->W, p->fmt, p->n, p->width, p->mue, p->raw_m, w, h, p->inv_pose,
+                                             p->K[0]);
     p->stage_ms[3] += now_ms() - t0;
     p->frame++;
     return status;
@@ -1081,6 +1092,19 @@ const real* KFO(pipeline_depth)(const pipe_t* p, int which, int level)
 }
 double KFO(pipeline_stage_ms)(const pipe_t* p, int stage) { return p->stage_ms[stage]; }
 int64_t KFO(pipeline_touched)(const pipe_t* p) { return p->touched; }
+/* Switches a fresh pipeline to the count-weight update (format 2) before its first frame. */
+int KFO(pipeline_use_count_weights)(pipe_t* p)
+{
+    if (p->frame != 0 || p->fmt != 0) return -1;
+    p->fmt = 2;
+    return 0;
+}
+/* Fuse with the count-weight update into F and W planes of `real`. */
+int64_t KFO(fuse_counts)(real* F, real* W, int n, real width, real mue, const real* depth, int w, int h,
+                         const real* inv_pose, const real* K)
+{
+    return fuse_impl(F, W, 2, n, width, mue, depth, w, h, inv_pose, K);
+}
 #ifndef KFREF_F64
 /* Switches a fresh pipeline to the precision form's volume (before the first frame). */
 int kfo_pipeline_use_snorm16(pipe_t* p)

@@ -108,6 +108,11 @@ const float* kfo_pipeline_depth(const kfo_pipeline* p, int which, int level);
 double kfo_pipeline_stage_ms(const kfo_pipeline* p, int stage);
 int64_t kfo_pipeline_touched(const kfo_pipeline* p);
 int    kfo_pipeline_use_snorm16(kfo_pipeline* p);
+/* count-weight update on unquantised planes (the precision form's rule without
+ * the snorm16 rounding); the FP64 build's kfo64_* twin is its error reference */
+int    kfo_pipeline_use_count_weights(kfo_pipeline* p);
+int64_t kfo_fuse_counts(float* F, float* W, int n, float width, float mue,
+                        const float* depth, int w, int h, const float inv_pose[16], const float K[9]);
 const int16_t* kfo_pipeline_tsdf_q(const kfo_pipeline* p);
 const uint8_t* kfo_pipeline_weight_q(const kfo_pipeline* p);
 

@@ -85,6 +85,8 @@ def _declare(l):
         fn("pipeline_depth", P, P, I, I)
         fn("pipeline_stage_ms", D, P, I)
         fn("pipeline_touched", I64, P)
+        fn("pipeline_use_count_weights", I, P)
+        fn("fuse_counts", I64, P, P, I, R, R, P, I, I, P, P)
     l.kfo_fuse_q.restype = I64
     l.kfo_fuse_q.argtypes = [P, P, I, F, F, P, I, I, P, P]
     l.kfo_initialize_grid_q.restype = None
@@ -205,6 +207,13 @@ class Oracle:
                              _p(np.ascontiguousarray(Kinv, np.float32)), w, h, _p(vout), _p(nout))
         return vout, nout
 
+    def fuse_counts(self, F, W, n, width, mue, depth, inv_pose, K):
+        """Count-weight update (the precision form's rule) on unquantised planes."""
+        depth = self.arr(depth)
+        h, w = depth.shape
+        return self.f("fuse_counts")(_p(F), _p(W), n, width, mue, _p(depth), w, h,
+                                     _p(self.arr(inv_pose)), _p(self.arr(K)))
+
     def fuse(self, F, W, n, width, mue, depth, inv_pose, K):
         depth = self.arr(depth)
         h, w = depth.shape
@@ -287,7 +296,8 @@ class OraclePipeline:
     """The reference's per-frame loop (kinect_fusion.cpp:44-183) on the CPU."""
 
     def __init__(self, w, h, n, width, mue, fx, fy, cx, cy, depth_scale, iters, dist_th, angle_th,
-                 stop_angle=0.0, stop_dist=0.0, init_pose=None, dtype=np.float32, snorm16=False):
+                 stop_angle=0.0, stop_dist=0.0, init_pose=None, dtype=np.float32, snorm16=False,
+                 count_weights=False):
         self.o = Oracle(dtype)
         self.w, self.h, self.n = w, h, n
         it = np.array(iters, np.int32)
@@ -301,6 +311,9 @@ class OraclePipeline:
             assert self.o.dt == np.float32
             if self.o.l.kfo_pipeline_use_snorm16(self.h_) != 0:
                 raise MemoryError("oracle precision-form volume allocation failed")
+        elif count_weights:
+            if self.o.f("pipeline_use_count_weights")(self.h_) != 0:
+                raise RuntimeError("oracle count-weight switch failed")
 
     def close(self):
         if self.h_:

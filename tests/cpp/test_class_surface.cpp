@@ -5,8 +5,9 @@
 //
 // Deviations from the reference tests: TestComputeNormal uses valid vertices
 // (the reference leaves the validity flag 0 and asserts only component [0]);
-// TestComputePose uses two frames of a seeded synthetic sequence instead of the
-// TUM freiburg1_xyz frames the reference loads from disk.
+// TestComputePose uses frames 0 and 4 of a seeded synthetic sequence instead of the
+// TUM freiburg1_xyz frames the reference loads from disk, and adds bounds below
+// the true motion so that a no-op estimate fails.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -130,19 +131,21 @@ static void TestConstructIcpResiduals()
 
 static void TestComputePose()
 {
-    // icp_tests.cpp:233-326 on two frames of a synthetic sequence (seed 1):
-    // rotation within pi/90 and translation within 1 cm of the ground truth.
-    const int W = 640, H = 480, frames = 2;
-    std::vector<double> poses(frames * 16), sph(8 * 4);
-    CHECK(kfs_trajectory(1, frames, poses.data()) == 0);
-    CHECK(kfs_spheres(1, 3.0, poses.data(), frames, 8, sph.data()) >= 0);
-    std::vector<uint16_t> raw((size_t)frames * W * H);
-    kfs_render(3.0, sph.data(), 8, poses.data(), 0, frames, W, H, 525, 525, 319.5, 239.5, 1, 0.0012, 0.0,
-               raw.data(), nullptr);
+    // icp_tests.cpp:233-326 on frames 0 and 4 of a 5-frame synthetic sequence (seed 1) instead of
+    // two TUM frames: frame 4's camera-space maps aligned against frame 0's from the identity.
+    // The reference's bounds (rotation within pi/90, translation within 1 cm per axis) plus bounds
+    // below the true motion (2.1 deg, 2.6 cm): an ICP that returns its start pose fails both.
+    const int W = 640, H = 480, n_frames = 5, used[2] = {0, 4};
+    std::vector<double> poses(n_frames * 16), sph(8 * 4);
+    CHECK(kfs_trajectory(1, n_frames, poses.data()) == 0);
+    CHECK(kfs_spheres(1, 3.0, poses.data(), n_frames, 8, sph.data()) >= 0);
+    std::vector<uint16_t> raw((size_t)2 * W * H);
+    for (int k = 0; k < 2; ++k)
+        kfs_render(3.0, sph.data(), 8, poses.data(), used[k], 1, W, H, 525, 525, 319.5, 239.5, 1, 0.0012, 0.0,
+                   raw.data() + (size_t)k * W * H, nullptr);
     IcpConfig icp_config{{8, 4, 2}, 640, 480, 0.5f, 6.0f * pi / 18.0f};
     std::vector<std::unique_ptr<GridMapPyramid<GridMap>>> vp, np;
-    Sensor sensors[2] = {Sensor(525.0f), Sensor(525.0f)};
-    for (int f = 0; f < frames; ++f) {
+    for (int f = 0; f < 2; ++f) {
         DepthMap raw_map(W, H);
         raw_map.update(raw.data() + (size_t)f * W * H);
         GridMap meters(W, H, MapFormat::Depth32);
@@ -153,18 +156,15 @@ static void TestComputePose()
         kernel::downSample(dp[1], dp[2]);
         vp.push_back(std::make_unique<GridMapPyramid<GridMap>>(W, H, 3, MapFormat::Vector4));
         np.push_back(std::make_unique<GridMapPyramid<GridMap>>(W, H, 3, MapFormat::Vector4));
-        Mat4 P;
-        for (int i = 0; i < 16; ++i) P[i] = (float)poses[f * 16 + i];
-        sensors[f].setPose(P);
+        Sensor sensor(525.0f);
         for (int l = 0; l < 3; ++l) {
-            kernel::createVertexMap(dp[l], (*vp[f])[l], sensors[f].getInverseIntr(l));
+            kernel::createVertexMap(dp[l], (*vp[f])[l], sensor.getInverseIntr(l));
             kernel::createNormalMap((*vp[f])[l], (*np[f])[l]);
         }
     }
-    // the reference aligns frame 2 against frame 1's camera-space maps from the identity (icp_tests.cpp:304-313)
-    // true relative pose = inverse(P1) * P2
-    double P1[16], P2[16];
-    for (int i = 0; i < 16; ++i) { P1[i] = poses[i]; P2[i] = poses[16 + i]; }
+    // true relative pose = inverse(P0) * P4 = [R0^T R4 | R0^T (t4 - t0)]
+    const double* P1 = &poses[used[0] * 16];
+    const double* P2 = &poses[used[1] * 16];
     double rel[3][4];
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 4; ++c) {
@@ -183,11 +183,24 @@ static void TestComputePose()
             a[r] += rel[r][q] * v[q];
             b[r] += est.rot_mat[q * 3 + r] * v[q];
         }
-    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    double na = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    double angle = std::acos(std::fmin(1.0, dot / (na * nb)));
+    auto angle_of = [](const double* x, const double* y) {
+        double dot = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+        double nx = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), ny = std::sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        return std::acos(std::fmin(1.0, dot / (nx * ny)));
+    };
+    const double angle = angle_of(a, b), true_angle = angle_of(a, v);
+    double true_shift = 0, err_shift = 0;
+    for (int i = 0; i < 3; ++i) {
+        true_shift = std::fmax(true_shift, std::fabs(rel[i][3]));
+        err_shift = std::fmax(err_shift, std::fabs(rel[i][3] - est.transl_vec[i]));
+    }
+    std::printf("  true motion %.5f rad / %.5f m, error %.5f rad / %.5f m\n", true_angle, true_shift, angle,
+                err_shift);
     CHECK(angle <= pi / 90.0f);
     for (int i = 0; i < 3; ++i) CHECK_NEAR(rel[i][3], est.transl_vec[i], 0.01);
+    CHECK(true_shift > 0.02 && true_angle > 0.02);  // the start pose fails the bounds below
+    CHECK(angle <= 0.2 * true_angle);
+    CHECK(err_shift <= 0.1 * true_shift);
     auto t = icp.getExecutionTimes();
     CHECK(t[0] > 0.0f && t[1] > 0.0f);
 }

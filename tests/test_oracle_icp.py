@@ -221,3 +221,98 @@ def test_compute_pose_identity_on_identical_maps(oracle32):
     assert np.abs(r["sums"][:, 21:]).max() < 1e-6 * np.abs(r["sums"][:, :21]).max()
     np.testing.assert_allclose(r["R"], I3, atol=1e-6)
     np.testing.assert_allclose(r["t"], z, atol=1e-6)
+
+
+def _exact_world_maps(seq_args, f, levels=3):
+    """World-space vertex and normal pyramids of frame f taken from the synthetic scene itself
+    (noiseless FP64 depth through the pixel centres, analytic plane / sphere normals oriented away
+    from the camera like the raycast's -grad F), at the pyramid's intrinsics f / 2^L,
+    c_L = (c + 1/2) / 2^L - 1/2."""
+    from kfusion.synth import SyntheticSequence
+    V, N = [], []
+    for lvl in range(levels):
+        s = 1.0 / (1 << lvl)
+        w, h = seq_args["width"] >> lvl, seq_args["height"] >> lvl
+        q = SyntheticSequence(seed=seq_args["seed"], n_frames=seq_args["n_frames"], L=seq_args["L"], width=w,
+                              height=h, fx=525.0 * s, fy=525.0 * s, cx=320.0 * s - 0.5, cy=240.0 * s - 0.5,
+                              noise_k=0.0)
+        _, td = q.frames(f, 1, true_depth=True)
+        z = td[0]
+        P = q.poses[f].reshape(4, 4).T
+        ii, jj = np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5)
+        ray = np.stack([(ii - q.cx) / q.fx, (jj - q.cy) / q.fy, np.ones_like(ii)], -1)
+        pw = (ray * z[..., None]) @ P[:3, :3].T + P[:3, 3]
+        a = 0.45 * q.L
+        resid = [np.abs(pw[..., 1] - a), np.abs(pw[..., 0] - a), np.abs(pw[..., 0] + a), np.abs(pw[..., 2] - a)]
+        resid += [np.abs(np.linalg.norm(pw - c[:3], axis=-1) - c[3]) for c in q.spheres]
+        which = np.argmin(np.stack(resid, -1), -1)
+        nw = np.zeros_like(pw)
+        for k, axis in enumerate(([0, 1.0, 0], [1.0, 0, 0], [-1.0, 0, 0], [0, 0, 1.0])):
+            nw[which == k] = axis
+        for k, c in enumerate(q.spheres):
+            m = which == 4 + k
+            nw[m] = (pw[m] - c[:3]) / c[3]
+        nw[np.sum(nw * (pw - P[:3, 3]), -1) < 0] *= -1
+        flag = np.where(z > 0, -1.0, -2.0)
+        V.append(np.concatenate([pw, flag[..., None]], -1).astype(np.float32))
+        N.append(np.concatenate([nw, flag[..., None]], -1).astype(np.float32))
+    return V, N
+
+
+def _measured_vertex_pyramid(o, raw):
+    d = o.convert_to_depth_meters(raw, 1e-3)
+    dp = [o.apply_bilateral_filter(d)]
+    for _ in range(2):
+        dp.append(o.down_sample(dp[-1]))
+    V, Ks = [], []
+    for lvl in range(3):
+        K, Ki = o.intrinsics(525.0, 525.0, 319.5, 239.5, lvl)
+        V.append(o.create_vertex_map(dp[lvl], Ki))
+        Ks.append(K)
+    return V, Ks
+
+
+@pytest.mark.parametrize("seed,L,n_frames,f", [(1, 3.0, 30, 0), (1, 3.0, 30, 7), (2, 4.0, 300, 0), (2, 4.0, 300, 5)])
+def test_compute_pose_recovers_known_motion(oracle32, seed, L, n_frames, f):
+    """icp.cpp:25-87 on a known rigid motion: the measured (noisy, filtered) pyramid of frame f + 1
+    against exact world-space targets of frame f, starting from the true pose of frame f, must land
+    on the true pose of frame f + 1 within 0.05 deg and 0.5 mm.  The start pose is 0.3-0.5 deg and
+    6-9 mm away, so an ICP that returns its start pose, solves for the wrong sign, or transposes
+    the update fails."""
+    from kfusion.synth import SyntheticSequence
+    seq = SyntheticSequence(seed=seed, n_frames=n_frames, L=L)
+    TV, TN = _exact_world_maps(dict(seed=seed, n_frames=n_frames, L=L, width=640, height=480), f)
+    V, Ks = _measured_vertex_pyramid(oracle32, seq.frames(f + 1, 1)[0])
+    P0, P1 = seq.poses[f].reshape(4, 4).T, seq.poses[f + 1].reshape(4, 4).T
+    Rin = np.ascontiguousarray(P0[:3, :3].T.reshape(-1), np.float32)
+    r = oracle32.compute_pose(V, TV, TN, [10, 5, 4], Ks, 0.1, float(np.deg2rad(20.0)), 0.0, 0.0, Rin,
+                              P0[:3, 3].astype(np.float32))
+    assert r["status"] == 0 and r["n_iters"] == 19
+
+    def angle_deg(Ra, Rb):
+        return float(np.degrees(np.arccos(np.clip((np.trace(Ra.T @ Rb) - 1) / 2, -1, 1))))
+
+    Re = r["R"].reshape(3, 3).T.astype(np.float64)
+    start_rot, start_trans = angle_deg(P1[:3, :3], P0[:3, :3]), np.linalg.norm(P1[:3, 3] - P0[:3, 3])
+    assert start_rot > 0.3 and start_trans > 5e-3  # the motion to recover
+    assert angle_deg(P1[:3, :3], Re) <= 0.05
+    assert np.linalg.norm(r["t"].astype(np.float64) - P1[:3, 3]) <= 5e-4
+
+
+def test_rigid_inverse_closed_form(oracle32, oracle64):
+    """sensor.h:30-34 (Sensor::setPose stores glm::inverse(pose)): for P = [R | t] the inverse is
+    [R^T | -R^T t]; checked against a general 4x4 inverse and P * P^-1 = I, in both precisions."""
+    rng = np.random.default_rng(17)
+    for _ in range(100):
+        a = rng.uniform(-np.pi, np.pi, 3)
+        Rm = rot("z", a[2]) @ rot("y", a[1]) @ rot("x", a[0])
+        t = rng.uniform(-3.0, 3.0, 3)
+        P = np.eye(4)
+        P[:3, :3], P[:3, 3] = Rm, t
+        for o, tol in ((oracle32, 5e-6), (oracle64, 1e-13)):
+            inv = o.rigid_inverse(Rm.T.reshape(-1), t).astype(np.float64).reshape(4, 4).T
+            Pq = np.eye(4)
+            Pq[:3, :3] = Rm.astype(o.dt)
+            Pq[:3, 3] = t.astype(o.dt)
+            np.testing.assert_allclose(inv, np.linalg.inv(Pq), atol=tol)
+            np.testing.assert_allclose(Pq @ inv, np.eye(4), atol=tol)
