@@ -334,6 +334,13 @@ int kf_batch_shape_get(kf_batch* b, kf_batch_shape* shape, int* tuned);
 /* GPU time of the ticks since the last call (events around each tick). */
 int kf_batch_tick_times(kf_batch* b, double* total_ms, int64_t* ticks, float* max_ms);
 int kf_batch_touched_total(kf_batch* b, unsigned long long* voxels);
+/* Per-kernel timing of the ticks enqueued while enabled: such ticks run as direct launches with
+ * events between the kernels (results are the same as from the tick's graph).  Enabling or
+ * disabling waits for the enqueued ticks and clears the totals.  total_ms: preprocess, raycast
+ * first pass, raycast finishing pass, ICP, fuse brick cull, fuse sweep, whole tick (ms summed
+ * over the timed ticks); ticks: their number. */
+int kf_batch_set_stage_timing(kf_batch* b, int enabled);
+int kf_batch_stage_times(kf_batch* b, double total_ms[7], int64_t* ticks);
 /* Session checkpoint (VoxelGrid::saveVoxelGrid/loadVoxelGrid, voxel_grid.cpp:43-80, plus the
  * pose): path holds the volume in the reference's raw n^3 x (f, w) float format (any voxel
  * format converts through it) and path + ".pose" the frame count and the camera -> world pose

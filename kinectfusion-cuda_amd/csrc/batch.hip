@@ -53,7 +53,10 @@ struct Slot {
     void* host = nullptr;  // pinned copy of the table
     void* dev = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // tick start, after each stage
+    // tick start, after each stage (preprocess, raycast, ICP, fuse); with per-kernel
+    // timing also after the raycast's first pass (5) and after the fuse's brick cull (6)
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool fine = false;     // the pending tick recorded events 5 and 6
     bool pending = false;  // a tick using this slot may still run
     bool staged = false;   // stage events were recorded (direct launch)
     bool tuning = false;   // the pending tick is a tuning sample
@@ -120,6 +123,10 @@ struct kf_batch {
     float tune_best[4][kCandidates];
     int tune_count[4][kCandidates];
     int tune_tick = 0;
+    // per-kernel timing (kf_batch_set_stage_timing)
+    bool stage_timing = false;
+    double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t stage_ticks = 0;
     // tick timing
     double tick_ms_total = 0.0;
     int64_t ticks_timed = 0;
@@ -246,7 +253,7 @@ int ensure_list(kf_batch* b)
 
 // The stages of one tick on the batch's stream.  ev (optional): 5 events
 // recorded at the tick's start and after each stage.
-int enqueue_tick(kf_batch* b, const Slot& sl, const kf_batch_shape& sh, hipEvent_t* ev)
+int enqueue_tick(kf_batch* b, const Slot& sl, const kf_batch_shape& sh, hipEvent_t* ev, bool fine = false)
 {
     const char* base = static_cast<const char*>(sl.dev);
     const auto* hd = reinterpret_cast<const kf::BatchHeader*>(base);
@@ -260,7 +267,7 @@ int enqueue_tick(kf_batch* b, const Slot& sl, const kf_batch_shape& sh, hipEvent
     int st;
     if ((st = kf::launch_preprocess_batch(hd, pre, sh.preprocess_blocks, s))) return st;
     if (ev) KF_HIP(hipEventRecord(ev[1], s));
-    if ((st = kf::launch_raycast_batch(hd, ray, sh.raycast_blocks, s))) return st;
+    if ((st = kf::launch_raycast_batch(hd, ray, sh.raycast_blocks, s, fine ? ev[5] : nullptr))) return st;
     if (ev) KF_HIP(hipEventRecord(ev[2], s));
     const kf_icp_config& ic = b->cfg.icp;
     if ((st = kf::launch_icp_batch(hd, icp, ic.iters_per_layer, ic.levels, b->cfg.max_sessions, ic.compensated,
@@ -268,7 +275,7 @@ int enqueue_tick(kf_batch* b, const Slot& sl, const kf_batch_shape& sh, hipEvent
                                    s)))
         return st;
     if (ev) KF_HIP(hipEventRecord(ev[3], s));
-    if ((st = kf::launch_fuse_batch(hd, fuse, sh.cull_blocks, sh.fuse_blocks, s))) return st;
+    if ((st = kf::launch_fuse_batch(hd, fuse, sh.cull_blocks, sh.fuse_blocks, s, fine ? ev[6] : nullptr))) return st;
     if (ev) KF_HIP(hipEventRecord(ev[4], s));
     return KF_OK;
 }
@@ -314,8 +321,18 @@ int retire(kf_batch* b, Slot& sl)
             b->tune_count[stg][sl.cand] += 1;
         }
     }
+    if (sl.fine) {
+        const int pairs[7][2] = {{0, 1}, {1, 5}, {5, 2}, {2, 3}, {3, 6}, {6, 4}, {0, 4}};
+        for (int k = 0; k < 7; ++k) {
+            float t = 0.0f;
+            KF_HIP(hipEventElapsedTime(&t, sl.ev[pairs[k][0]], sl.ev[pairs[k][1]]));
+            b->stage_ms[k] += t;
+        }
+        b->stage_ticks += 1;
+    }
     sl.pending = false;
     sl.tuning = false;
+    sl.fine = false;
     return KF_OK;
 }
 
@@ -692,6 +709,9 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
         sl.has_stage[2] = n_icp > 0;
         sl.has_stage[3] = n > 0;
         b->tune_tick += 1;
+    } else if (b->stage_timing) {
+        st = enqueue_tick(b, sl, b->shape, sl.ev, true);
+        sl.fine = true;
     } else if (b->cfg.use_graph) {
         if (!sl.exec) st = capture(b, sl);
         if (!st) {
@@ -707,6 +727,27 @@ int kf_batch_tick(kf_batch* b, int n, const int* ids, const uint16_t* const* raw
     sl.pending = true;
     b->next ^= 1;
     for (int k = 0; k < n; ++k) b->sess[ids[k]].frames += 1;
+    return KF_OK;
+}
+
+int kf_batch_set_stage_timing(kf_batch* b, int enabled)
+{
+    if (!b) return kf::fail(KF_ERR_ARG, "kf_batch_set_stage_timing: null batch");
+    int st = kf_batch_sync(b);
+    if (st) return st;
+    b->stage_timing = enabled != 0;
+    for (double& v : b->stage_ms) v = 0.0;
+    b->stage_ticks = 0;
+    return KF_OK;
+}
+
+int kf_batch_stage_times(kf_batch* b, double total_ms[7], int64_t* ticks)
+{
+    if (!b || !total_ms) return kf::fail(KF_ERR_ARG, "kf_batch_stage_times: null argument");
+    int st = kf_batch_sync(b);
+    if (st) return st;
+    for (int k = 0; k < 7; ++k) total_ms[k] = b->stage_ms[k];
+    if (ticks) *ticks = b->stage_ticks;
     return KF_OK;
 }
 

@@ -89,8 +89,9 @@ struct FuseJob {
 };
 FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const float* K, const float* inv_pose,
                    const PoseState* dyn_state);
+// mid (optional): an event recorded between the brick cull and the sweep.
 int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_blocks, unsigned list_blocks,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t mid = nullptr);
 // inv_pose: by-value world->camera matrix; dyn_state (may be null) overrides it
 // with PoseState::inv_pose and skips the launch's work when the frame failed.
 // list_events (optional): two events recorded around the brick-list kernel alone.
@@ -135,7 +136,9 @@ struct RaycastJob {
     RaycastLevel L;
 };
 RaycastVolume raycast_volume(const kf_volume* v);
-int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s);
+// mid (optional): an event recorded between the first pass and the finishing pass.
+int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s,
+                         hipEvent_t mid = nullptr);
 int launch_raycast(const kf_volume* v, const float* pose, const PoseState* dyn_state, const float* inv_k,
                    float4* vout, float4* nout, int w, int h, hipStream_t s);
 // All pyramid levels in one launch (level l is w0 >> l by h0 >> l).

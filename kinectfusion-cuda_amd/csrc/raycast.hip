@@ -981,14 +981,17 @@ RaycastVolume raycast_volume(const kf_volume* v)
     return r;
 }
 
-int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s)
+// The two voxel formats use disjoint tile counters, ray-list halves and maps, so
+// both first passes run before both finishing passes.
+int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned blocks, hipStream_t s, hipEvent_t mid)
 {
     raycast_batch_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
-    raycast_batch_finish_kernel<KF_VOXEL_F32><<<kFinishWaves, 64, 0, s>>>(hd, jobs);
-    KF_LAUNCHED("raycast_batch_finish_kernel");
     raycast_batch_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
+    if (mid) KF_HIP(hipEventRecord(mid, s));
+    raycast_batch_finish_kernel<KF_VOXEL_F32><<<kFinishWaves, 64, 0, s>>>(hd, jobs);
+    KF_LAUNCHED("raycast_batch_finish_kernel");
     raycast_batch_finish_kernel<KF_VOXEL_SNORM16_U8><<<kFinishWaves, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_finish_kernel");
     return KF_OK;

@@ -653,10 +653,11 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
 }
 
 int launch_fuse_batch(const BatchHeader* hd, const FuseJob* jobs, unsigned cull_blocks, unsigned list_blocks,
-                      hipStream_t s)
+                      hipStream_t s, hipEvent_t mid)
 {
     fuse_batch_cull_kernel<<<cull_blocks, kBatchCullThreads, 0, s>>>(hd, jobs);
     KF_LAUNCHED("fuse_batch_cull_kernel");
+    if (mid) KF_HIP(hipEventRecord(mid, s));
     fuse_batch_list_kernel<<<list_blocks, 64 * kFuseWaves, 0, s>>>(hd, jobs);
     KF_LAUNCHED("fuse_batch_list_kernel");
     return KF_OK;

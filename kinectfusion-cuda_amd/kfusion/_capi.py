@@ -240,6 +240,8 @@ SIGNATURES = {
     "kf_batch_shape_get": (I, [P, C.POINTER(kf_batch_shape), C.POINTER(I)]),
     "kf_batch_tick_times": (I, [P, C.POINTER(C.c_double), C.POINTER(I64), FP]),
     "kf_batch_touched_total": (I, [P, C.POINTER(C.c_ulonglong)]),
+    "kf_batch_set_stage_timing": (I, [P, I]),
+    "kf_batch_stage_times": (I, [P, P, C.POINTER(I64)]),
     "kf_batch_session_save": (I, [P, I, C.c_char_p]),
     "kf_batch_session_restore": (I, [P, I, C.c_char_p]),
     "kf_server_checkpoint": (I, [P, I, C.c_char_p]),

@@ -191,6 +191,15 @@ int kfo_threads(void)
     return 1;
 #endif
 }
+/* Sets the threads of the following parallel loops (the CPU baseline's single-thread figure). */
+void kfo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 #endif
 static inline real kf_exp(real x)
 {
@@ -477,6 +486,11 @@ static inline real voxel_value(const void* F, int fmt, int64_t q)
 {
     return fmt != 1 ? ((const real*)F)[q] : snorm_decode(((const int16_t*)F)[q]);
 }
+/* Footprint recording (measurement only, kfo_raycast_footprint): when set, every
+ * trilinear sample marks its 8 corner voxels in this n^3 byte map and counts
+ * itself.  Concurrent marks all store 1, so the map needs no ordering. */
+static uint8_t* g_footprint = NULL;
+static int64_t g_footprint_samples = 0;
 static real sample_any(const void* F, int fmt, int n, real u, real v, real s)
 {
     real x = u * (real)n - R(0.5), y = v * (real)n - R(0.5), z = s * (real)n - R(0.5);
@@ -490,6 +504,14 @@ static real sample_any(const void* F, int fmt, int n, real u, real v, real s)
     int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
     int i1 = clampi(i0 + 1, 0, n - 1), j1 = clampi(j0 + 1, 0, n - 1), k1 = clampi(k0 + 1, 0, n - 1);
     i0 = clampi(i0, 0, n - 1); j0 = clampi(j0, 0, n - 1); k0 = clampi(k0, 0, n - 1);
+    if (g_footprint) {
+        g_footprint[vidx(n, i0, j0, k0)] = 1; g_footprint[vidx(n, i1, j0, k0)] = 1;
+        g_footprint[vidx(n, i0, j1, k0)] = 1; g_footprint[vidx(n, i1, j1, k0)] = 1;
+        g_footprint[vidx(n, i0, j0, k1)] = 1; g_footprint[vidx(n, i1, j0, k1)] = 1;
+        g_footprint[vidx(n, i0, j1, k1)] = 1; g_footprint[vidx(n, i1, j1, k1)] = 1;
+#pragma omp atomic
+        g_footprint_samples += 1;
+    }
     real c000 = voxel_value(F, fmt, vidx(n, i0, j0, k0)), c100 = voxel_value(F, fmt, vidx(n, i1, j0, k0));
     real c010 = voxel_value(F, fmt, vidx(n, i0, j1, k0)), c110 = voxel_value(F, fmt, vidx(n, i1, j1, k0));
     real c001 = voxel_value(F, fmt, vidx(n, i0, j0, k1)), c101 = voxel_value(F, fmt, vidx(n, i1, j0, k1));
@@ -613,6 +635,31 @@ void kfo_raycast_q(const int16_t* F, int n, float width, float mue, const float*
                    kfo_float4* vout, kfo_float4* nout)
 {
     raycast_impl(F, 1, n, width, mue, pose, Ki, w, h, vout, nout);
+}
+/* The raycast's footprint (measurement only): runs the march of raycast_impl
+ * (fmt 0: float F, 1: snorm16 codes) with every sample -- march samples and
+ * the 6 gradient samples of each hit -- marking its trilinear corners in
+ * mark (n^3 bytes, zeroed by the caller, accumulated over calls).  Returns the
+ * number of samples taken.  Not reentrant. */
+int64_t kfo_raycast_footprint(const void* F, int fmt, int n, float width, float mue, const float* pose, const float* Ki,
+                              int w, int h, uint8_t* mark)
+{
+    kfo_float4* maps = (kfo_float4*)malloc(sizeof(kfo_float4) * 2 * (size_t)w * h);
+    if (!maps) return -1;
+    g_footprint = mark;
+    g_footprint_samples = 0;
+    raycast_impl(F, fmt, n, width, mue, pose, Ki, w, h, maps, maps + (size_t)w * h);
+    g_footprint = NULL;
+    free(maps);
+    return g_footprint_samples;
+}
+/* Number of non-zero bytes of a footprint map. */
+int64_t kfo_count_marked(const uint8_t* mark, int64_t count)
+{
+    int64_t c = 0;
+#pragma omp parallel for schedule(static) reduction(+ : c)
+    for (int64_t q = 0; q < count; ++q) c += mark[q] != 0;
+    return c;
 }
 #endif
 

@@ -41,6 +41,7 @@ typedef struct { float x, y, z, w; } kfo_float4;
 float kfo_u16_to_half_float(uint16_t code);
 float kfo_expf(float x);
 int   kfo_threads(void);
+void  kfo_set_threads(int n);
 int   kfo_normals_too_different(const float n[3], const float tn[3], const float R[9], float cos_thresh);
 int   kfo_normals_too_different_acos(const float n[3], const float tn[3], const float R[9], float angle_thresh);
 void  kfo_correspondence(const float s[3], const float R_prev[9], const float t_prev[3], const float K[9], int uv[2], float fuv[2]);
@@ -71,6 +72,10 @@ void kfo_raycast(const float* F, int n, float width, float mue, const float pose
 float kfo_sample_tsdf(const float* F, int n, float u, float v, float s);
 void kfo_raycast_q(const int16_t* F, int n, float width, float mue, const float pose[16], const float K_inv[9],
                    int w, int h, kfo_float4* vout, kfo_float4* nout);
+/* measurement only: the distinct voxels the reference march reads (fmt 0 float, 1 snorm16) */
+int64_t kfo_raycast_footprint(const void* F, int fmt, int n, float width, float mue, const float pose[16],
+                              const float K_inv[9], int w, int h, uint8_t* mark);
+int64_t kfo_count_marked(const uint8_t* mark, int64_t count);
 
 /* ---- ICP (project/src/icp.cu, icp.cuh, icp.cpp, linear_least_squares.cpp) ---- */
 void kfo_construct_icp_residuals(const kfo_float4* vmap, const kfo_float4* tvmap, const kfo_float4* tnmap,

@@ -93,6 +93,10 @@ def _declare(l):
     l.kfo_initialize_grid_q.argtypes = [P, P, I, F, F]
     l.kfo_raycast_q.restype = None
     l.kfo_raycast_q.argtypes = [P, I, F, F, P, P, I, I, P, P]
+    l.kfo_raycast_footprint.restype = I64
+    l.kfo_raycast_footprint.argtypes = [P, I, I, F, F, P, P, I, I, P]
+    l.kfo_count_marked.restype = I64
+    l.kfo_count_marked.argtypes = [P, I64]
     l.kfo_pipeline_use_snorm16.restype = I
     l.kfo_pipeline_use_snorm16.argtypes = [P]
     l.kfo_pipeline_tsdf_q.restype = P
@@ -101,6 +105,8 @@ def _declare(l):
     l.kfo_pipeline_weight_q.argtypes = [P]
     l.kfo_threads.restype = I
     l.kfo_threads.argtypes = []
+    l.kfo_set_threads.restype = None
+    l.kfo_set_threads.argtypes = [I]
     l.kfo_expf.restype = F
     l.kfo_expf.argtypes = [F]
     l.kfo_cholesky_solve6.restype = I
@@ -230,6 +236,22 @@ class Oracle:
         self.f("raycast")(_p(F), n, width, mue, _p(self.arr(pose)), _p(self.arr(Kinv)), w, h,
                           _p(vout), _p(nout))
         return vout, nout
+
+    def raycast_footprint(self, F, n, width, mue, pose, Kinv_levels, w, h):
+        """(distinct voxels, samples) of the reference march over the levels whose inverse
+        intrinsics are given (level l: w >> l by h >> l); F float32 or int16 snorm16 codes."""
+        F = np.ascontiguousarray(F)
+        fmt = 1 if F.dtype == np.int16 else 0
+        if fmt == 0:
+            F = np.ascontiguousarray(F, np.float32)
+        mark = np.zeros(n ** 3, np.uint8)
+        samples = 0
+        for lvl, Ki in enumerate(Kinv_levels):
+            samples += self.l.kfo_raycast_footprint(_p(F), fmt, n, width, mue,
+                                                    _p(np.ascontiguousarray(pose, np.float32)),
+                                                    _p(np.ascontiguousarray(Ki, np.float32)), w >> lvl, h >> lvl,
+                                                    _p(mark))
+        return int(self.l.kfo_count_marked(_p(mark), mark.size)), int(samples)
 
     def sample_tsdf(self, F, n, u, v, s):
         return self.f("sample_tsdf")(_p(F), n, u, v, s)

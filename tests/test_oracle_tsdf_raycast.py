@@ -149,3 +149,31 @@ def test_fp64_oracle_close_to_fp32(oracle32, oracle64):
     seen = (W32 > 0) & (W64 > 0)
     assert seen.sum() > 1000
     np.testing.assert_allclose(F32[seen], F64[seen], atol=1e-5)
+
+
+def test_raycast_footprint_counts(oracle32):
+    """The raycast footprint (bench.py's algorithmic bytes of the raycast): over free space every
+    ray takes only large steps, so |S| grows with the image; a fused wall adds the small steps and
+    gradients near the surface; recording never changes the maps."""
+    from gpu_util import small_sequence
+    n, L, mue, w, h = 32, 2.0, 0.1, 16, 12
+    K, Ki = oracle32.intrinsics(16.0, 16.0, 7.5, 5.5, 0)
+    F, W = oracle32.new_volume(n)
+    F[:] = 1.0
+    pose = np.eye(4, dtype=np.float32).reshape(-1)
+    d_free, s_free = oracle32.raycast_footprint(F, n, L, mue, pose, [Ki], w, h)
+    assert s_free > 0 and 0 < d_free <= 8 * s_free
+    d_small, s_small = oracle32.raycast_footprint(F, n, L, mue, pose, [Ki], 4, 3)
+    assert d_small < d_free and s_small < s_free
+    depth = np.full((h, w), 0.7, np.float32)
+    F2, W2 = oracle32.new_volume(n)
+    oracle32.fuse(F2, W2, n, L, mue, depth, pose, K)
+    v0, n0 = oracle32.raycast(F2, n, L, mue, pose, Ki, w, h)
+    d_wall, s_wall = oracle32.raycast_footprint(F2, n, L, mue, pose, [Ki], w, h)
+    v1, n1 = oracle32.raycast(F2, n, L, mue, pose, Ki, w, h)
+    assert (v0[..., 3] == -1).mean() > 0.5
+    np.testing.assert_array_equal(v0, v1)
+    np.testing.assert_array_equal(n0, n1)
+    assert s_wall > 0 and d_wall <= 8 * s_wall
+    codes = np.where(F2 > 1.0, -32768, np.rint(np.clip(F2, -1, 1) * 32767)).astype(np.int16)
+    assert oracle32.raycast_footprint(codes, n, L, mue, pose, [Ki], w, h) == (d_wall, s_wall)
