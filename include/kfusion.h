@@ -164,6 +164,15 @@ int kf_construct_icp_residuals(const float* vertex_map, const float* target_vert
  * result: 6 floats on the host.  KF_ERR_NOT_PD when A^T A is not positive definite. */
 int kf_solve_normal_equations(const float* mat_a, const float* vec_b, unsigned int n_equations, float result[6],
                               kf_stream_t stream, float* elapsed_ms);
+/* kf_solve_normal_equations without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).  It only
+ * enqueues on stream: result_dev (6 floats, device) receives the solution and status_dev (one int,
+ * device, optional) KF_OK or KF_ERR_NOT_PD / KF_ERR_NONFINITE, exactly as kf_solve_normal_equations
+ * computes them.  Its working memory is a fixed device array of about 1 MiB per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap on one GPU race.
+ * n_equations <= 2^22 (KF_ERR_ARG above). */
+int kf_solve_normal_equations_noalloc(const float* mat_a, const float* vec_b, unsigned int n_equations,
+                                      float* result_dev, int* status_dev, kf_stream_t stream);
 /* One ICP iteration's association fused with its reduction: the 27 sums of the
  * rows kf_construct_icp_residuals would produce, plus the number of accepted
  * rows, without materialising A and b.  sums_host: 21 upper-triangle entries of

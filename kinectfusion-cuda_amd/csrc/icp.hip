@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "kf_device.hpp"
@@ -1324,7 +1325,58 @@ int launch_icp_batch(const BatchHeader* hd, const IcpJob* jobs, const int* iters
     return KF_OK;
 }
 
+// ---- LinearLeastSquares::solve without allocation (kf_solve_normal_equations_noalloc).
+// Its working memory -- the PoseState the solve writes and the per-block partial
+// rows -- lives in one file-scope device array, `scratch`, of kNoAllocBytes on
+// every GPU that runs it (held until the process ends).  Host code reaches a
+// device's copy only through the address hipGetSymbolAddress returns for it
+// (noalloc_scratch); the array's name is a host-side stand-in, not a device
+// address.  Every call on a GPU shares that GPU's copy.
+constexpr unsigned kNoAllocMaxEquations = 1u << 22;
+constexpr int kNoAllocMaxBlocks = (int)((kNoAllocMaxEquations + kThreads * kPixPerThread - 1) /
+                                        (kThreads * kPixPerThread));
+constexpr size_t kNoAllocStateBytes = (sizeof(PoseState) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes = kNoAllocStateBytes + sizeof(float) * kPartialStride * kNoAllocMaxBlocks;
+
+__global__ void noalloc_clear_state_kernel(PoseState* st)
+{
+    int* w = reinterpret_cast<int*>(st);
+    for (size_t q = threadIdx.x; q < sizeof(PoseState) / sizeof(int); q += blockDim.x) w[q] = 0;
+}
+
+__global__ void noalloc_result_kernel(const PoseState* st, float* x, int* status)
+{
+    if (threadIdx.x < 6) x[threadIdx.x] = st->last_x[threadIdx.x];
+    if (threadIdx.x == 0 && status) *status = st->status;
+}
+
 }  // namespace kf
+
+static __device__ __align__(256) unsigned char scratch[kf::kNoAllocBytes];
+
+namespace {
+
+// The device address of this GPU's `scratch`, looked up once per device.
+int noalloc_scratch(char** out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static char* ptr[kMaxDevices] = {};
+    int dev = 0;
+    KF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return kf::fail(KF_ERR_ARG, "solve (no allocation): device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return kf::fail_hip(e, "solve (no allocation): scratch address");
+        ptr[dev] = static_cast<char*>(p);
+    }
+    *out = ptr[dev];
+    return KF_OK;
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------ C ABI
 extern "C" {
@@ -1416,6 +1468,28 @@ int kf_solve_normal_equations(const float* mat_a, const float* vec_b, unsigned i
     (void)hipFree(st);
     (void)hipFree(partials);
     return rc;
+}
+
+int kf_solve_normal_equations_noalloc(const float* mat_a, const float* vec_b, unsigned int n_equations,
+                                      float* result_dev, int* status_dev, kf_stream_t stream)
+{
+    if (!mat_a || !vec_b || !result_dev || n_equations == 0)
+        return kf::fail(KF_ERR_ARG, "solve (no allocation): bad argument");
+    if (n_equations > kf::kNoAllocMaxEquations)
+        return kf::fail(KF_ERR_ARG, "solve (no allocation): more than 2^22 equations");
+    char* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    auto* st = reinterpret_cast<kf::PoseState*>(base);
+    auto* partials = reinterpret_cast<float*>(base + kf::kNoAllocStateBytes);
+    hipStream_t s = kf::stream_of(stream);
+    const int blocks = kf::blocks_for(n_equations);
+    kf::noalloc_clear_state_kernel<<<1, 256, 0, s>>>(st);
+    kf::ab_reduce_kernel<false><<<blocks, kf::kThreads, 0, s>>>(mat_a, vec_b, n_equations, partials, blocks);
+    kf::icp_finalize_kernel<false><<<1, kf::kThreads, 0, s>>>(st, partials, blocks, blocks, 0, 0.0f, 0.0f, 1);
+    kf::noalloc_result_kernel<<<1, 64, 0, s>>>(st, result_dev, status_dev);
+    KF_HIP(hipGetLastError());
+    return KF_OK;
 }
 
 int kf_icp_reduce(const float* vertex_map, const float* target_vertex_map, const float* target_normal_map, int w,

@@ -202,6 +202,7 @@ SIGNATURES = {
     "kf_volume_load": (I, [P, C.c_char_p]),
     "kf_construct_icp_residuals": (I, [P, P, P, I, I, P, P, P, P, P, F, F, P, P, P, FP]),
     "kf_solve_normal_equations": (I, [P, P, C.c_uint, P, P, FP]),
+    "kf_solve_normal_equations_noalloc": (I, [P, P, C.c_uint, P, P, P]),
     "kf_icp_reduce": (I, [P, P, P, I, I, P, P, P, P, P, F, F, I, P, C.POINTER(I64), P, FP]),
     "kf_icp_create": (I, [C.POINTER(kf_icp_config), C.POINTER(P)]),
     "kf_icp_destroy": (I, [P]),
