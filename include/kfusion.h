@@ -363,13 +363,17 @@ int kf_batch_session_restore(kf_batch* b, int session, const char* path);
  * (queued FIFO until slots and memory free up), every admitted volume stays
  * resident, frames are queued per session as they arrive and each tick takes
  * the oldest waiting frame of every session (continuous batching, at most two
- * ticks in flight).  Per-frame latency = completion seen by the loop - arrival.
+ * ticks in flight; launch_fill and max_wait_ms decide when a tick starts).  Per-frame latency = completion seen by the loop - arrival.
  * No reference counterpart (the reference serves one camera per process). */
 typedef struct kf_server_config {
     kf_batch_config batch;       /* max_sessions = resident session slots */
     size_t hbm_budget_bytes;     /* admission budget: each resident session is charged kf_session_bytes
-                                    plus its device frame buffer (2 bytes per pixel) */
+                                    plus its two device frame buffers (2 x 2 bytes per pixel) */
     int queue_depth;             /* frames buffered per session (pinned host memory); 0 = 8 */
+    float launch_fill;           /* a tick is launched once frames of at least this fraction of the resident
+                                    sessions wait; 0 = 0.9 */
+    float max_wait_ms;           /* ...or, with no tick running, once the oldest waiting frame has waited this
+                                    long; 0 = 2 ms, < 0 = never wait (launch whenever the GPU is idle) */
 } kf_server_config;
 typedef struct kf_server_stats {
     int64_t frames, ticks, admitted, queued_admissions, failed_frames;

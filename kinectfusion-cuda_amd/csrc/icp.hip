@@ -163,15 +163,6 @@ __device__ __forceinline__ bool icp_row(const IcpArgs& a, int u, int v, const fl
     return icp_pixel_row(a, icp_pixel_in(a, u, v), Rc, tc, M, tp, row, b);
 }
 
-// s += v with the rounding error of the addition carried in c (TwoSum).
-__device__ __forceinline__ void comp_add(float& s, float& c, float v)
-{
-    float t = s + v;
-    float bp = t - s;
-    float e = (s - (t - bp)) + (v - bp);
-    s = t;
-    c = c + e;
-}
 // (s1, c1) += (s2, c2)
 __device__ __forceinline__ void comp_merge(float& s1, float& c1, float s2, float c2)
 {
@@ -183,24 +174,22 @@ __device__ __forceinline__ void comp_merge(float& s1, float& c1, float s2, float
 }
 
 template <bool COMP>
-// Adds a pixel's row to the lane's sums of A^T A (upper triangle) and A^T b.
-// The plain form fuses each product into its sum (one rounding instead of two);
-// the compensated form adds the rounded product with its error carried.
+// Adds a pixel's row to the lane's sums of A^T A (upper triangle) and A^T b,
+// each product fused into its sum (one rounding instead of two).  A lane sums
+// at most kPixPerThread rows, so its few roundings are small against the
+// total; the compensated form carries the rounding errors of every addition
+// after that (lanes, waves, workgroups: block_reduce_store and the partial-row
+// sums), where nearly all of the additions happen.
 __device__ __forceinline__ void accumulate(float* s, float* c, const float* row, float b)
 {
+    (void)c;
     int q = 0;
 #pragma unroll
     for (int p = 0; p < 6; ++p)
 #pragma unroll
-        for (int r = p; r < 6; ++r, ++q) {
-            if (COMP) comp_add(s[q], c[q], row[p] * row[r]);
-            else s[q] = fmaf(row[p], row[r], s[q]);
-        }
+        for (int r = p; r < 6; ++r, ++q) s[q] = fmaf(row[p], row[r], s[q]);
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        if (COMP) comp_add(s[21 + p], c[21 + p], row[p] * b);
-        else s[21 + p] = fmaf(row[p], b, s[21 + p]);
-    }
+    for (int p = 0; p < 6; ++p) s[21 + p] = fmaf(row[p], b, s[21 + p]);
 }
 
 // One halving step of the lane reduction: the first n/2 entries of s (and c)

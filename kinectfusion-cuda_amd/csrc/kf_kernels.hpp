@@ -64,6 +64,7 @@ struct FuseArgs {
     const PoseState* dyn;  // when set: inv_pose comes from dyn->inv_pose, skip if dyn->frame_skip_fuse
     int n, nb;
     float half, res, mue;
+    float inv_mue;  // 1 / mue correctly rounded (the exact reciprocal the quotient diff / mue is formed from)
     const float* zk;
     void* f;
     void* wt;

@@ -8,6 +8,9 @@
 // most two ticks are in flight, so frames that arrive while the GPU works on
 // one tick are batched into the next: continuous batching.  Latency of a
 // frame = the loop clock when its tick is seen complete - its arrival stamp.
+// A tick is launched once enough sessions have a frame waiting, or, with the
+// GPU idle, once the oldest waiting frame has waited long enough
+// (kf_server_config::launch_fill, max_wait_ms).
 #include <math.h>
 #include <string.h>
 
@@ -30,7 +33,7 @@ struct Ticket {
     int state = 0;  // 0 waiting, 1 admitted, -1 released
     int sid = -1;
     bool closing = false;
-    uint16_t* raw_dev = nullptr;
+    uint16_t* raw_dev = nullptr;  // kInFlight device frame buffers, one per tick slot
     uint16_t* ring = nullptr;  // pinned, cap frames
     int cap = 0, head = 0, count = 0, inflight = 0;
     std::vector<double> arrival;
@@ -40,6 +43,7 @@ struct Ticket {
 
 struct Tick {
     hipEvent_t ev = nullptr;
+    hipEvent_t copied = nullptr;  // the tick's frame uploads (copy stream) are done
     bool busy = false;
     std::vector<int> tickets;
     std::vector<double> arrivals;
@@ -69,6 +73,7 @@ struct SplitMix64 {
 struct kf_server {
     kf_server_config cfg{};
     hipStream_t s = nullptr;
+    hipStream_t cs = nullptr;  // frame uploads, overlapping the running tick
     kf_batch* b = nullptr;
     std::vector<Ticket> t;
     std::deque<int> waiting;
@@ -108,7 +113,7 @@ int admit(kf_server* s)
         for (const Ticket& o : s->t) free_slots += (o.state == 1) ? 1 : 0;
         if (free_slots >= s->cfg.batch.max_sessions) return KF_OK;
         // the frame buffer first: a failure then leaves nothing half admitted
-        hipError_t e = hipMalloc(&q.raw_dev, q.frame_bytes);
+        hipError_t e = hipMalloc(&q.raw_dev, q.frame_bytes * kInFlight);
         if (e != hipSuccess) {
             q.raw_dev = nullptr;
             return kf::fail_hip(e, "server session buffer");
@@ -154,10 +159,33 @@ int retire(kf_server* s, Tick& k, bool wait)
     return KF_OK;
 }
 
-int launch(kf_server* s, int* launched)
+int launch(kf_server* s, int* launched, bool flush)
 {
     Tick& k = s->fl[s->next];
     if (k.busy) return KF_OK;
+    // A tick costs nearly as much with a few frames as with many, so ticks are
+    // launched full: once frames of launch_fill of the resident sessions wait
+    // (then it may also queue behind a running tick), or, with the GPU idle,
+    // once the oldest waiting frame has waited max_wait_ms.
+    bool running = false;
+    for (const Tick& t : s->fl) running = running || t.busy;
+    int resident = 0, waiting = 0;
+    double oldest = 0.0;
+    for (const Ticket& q : s->t) {
+        if (q.state != 1) continue;
+        resident += 1;
+        if (q.count - q.inflight > 0) {
+            const double a = q.arrival[(q.head + q.inflight) % q.cap];
+            oldest = waiting ? std::min(oldest, a) : a;
+            waiting += 1;
+        }
+    }
+    if (waiting == 0) return KF_OK;
+    const int need = std::max(1, (int)ceil((double)s->cfg.launch_fill * resident - 1e-9));
+    if (waiting < need && !flush) {
+        if (running) return KF_OK;
+        if (s->cfg.max_wait_ms > 0.0f && (clock_s() - oldest) * 1e3 < (double)s->cfg.max_wait_ms) return KF_OK;
+    }
     std::vector<int> sids;
     std::vector<const uint16_t*> raws;
     for (int id = 0; id < (int)s->t.size(); ++id) {
@@ -165,15 +193,20 @@ int launch(kf_server* s, int* launched)
         // the oldest frame not yet in a tick: ring position head + inflight
         if (q.state != 1 || q.count - q.inflight <= 0) continue;
         const int slot = (q.head + q.inflight) % q.cap;
-        KF_HIP(hipMemcpyAsync(q.raw_dev, q.ring + (size_t)slot * (q.frame_bytes / 2), q.frame_bytes,
-                              hipMemcpyHostToDevice, s->s));
+        // this tick slot's own device buffer: the tick that used it before has
+        // retired, and the running tick reads the other one
+        uint16_t* dst = q.raw_dev + (size_t)s->next * (q.frame_bytes / 2);
+        KF_HIP(hipMemcpyAsync(dst, q.ring + (size_t)slot * (q.frame_bytes / 2), q.frame_bytes, hipMemcpyHostToDevice,
+                              s->cs));
         sids.push_back(q.sid);
-        raws.push_back(q.raw_dev);
+        raws.push_back(dst);
         k.tickets.push_back(id);
         k.arrivals.push_back(q.arrival[slot]);
         q.inflight += 1;
     }
     if (sids.empty()) return KF_OK;
+    KF_HIP(hipEventRecord(k.copied, s->cs));
+    KF_HIP(hipStreamWaitEvent(s->s, k.copied, 0));
     int st = kf_batch_tick(s->b, (int)sids.size(), sids.data(), raws.data());
     if (st) return st;
     KF_HIP(hipEventRecord(k.ev, s->s));
@@ -189,6 +222,20 @@ Ticket* ticket_of(kf_server* s, int id)
 {
     if (!s || id < 0 || id >= (int)s->t.size()) return nullptr;
     return &s->t[id];
+}
+
+// One turn of the loop; flush launches whatever waits (draining: nothing more arrives).
+int loop_turn(kf_server* s, int* launched, bool flush)
+{
+    if (launched) *launched = 0;
+    int st = KF_OK;
+    for (Tick& k : s->fl)
+        if ((st = retire(s, k, false))) return st;
+    for (Ticket& q : s->t)
+        if (q.state == 1 && q.closing && q.count == 0 && q.inflight == 0)
+            if ((st = release(s, q))) return st;
+    if ((st = admit(s))) return st;
+    return launch(s, launched, flush);
 }
 
 double percentile(std::vector<double> v, double p)
@@ -211,14 +258,18 @@ int kf_server_destroy(kf_server* s)
 {
     if (!s) return KF_OK;
     if (s->s) (void)hipStreamSynchronize(s->s);
+    if (s->cs) (void)hipStreamSynchronize(s->cs);
     for (Ticket& q : s->t) {
         (void)hipFree(q.raw_dev);
         if (q.ring) (void)hipHostFree(q.ring);
     }
     kf_batch_destroy(s->b);
-    for (Tick& k : s->fl)
+    for (Tick& k : s->fl) {
         if (k.ev) (void)hipEventDestroy(k.ev);
+        if (k.copied) (void)hipEventDestroy(k.copied);
+    }
     if (s->s) (void)hipStreamDestroy(s->s);
+    if (s->cs) (void)hipStreamDestroy(s->cs);
     delete s;
     return KF_OK;
 }
@@ -230,10 +281,15 @@ int kf_server_create(const kf_server_config* cfg, kf_server** out)
     kf_server* s = new kf_server();
     s->cfg = *cfg;
     if (s->cfg.queue_depth <= 0) s->cfg.queue_depth = 8;
+    if (!(s->cfg.launch_fill > 0.0f)) s->cfg.launch_fill = 0.9f;
+    if (s->cfg.launch_fill > 1.0f) s->cfg.launch_fill = 1.0f;
+    if (s->cfg.max_wait_ms == 0.0f) s->cfg.max_wait_ms = 2.0f;
     hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->cs, hipStreamNonBlocking);
     int st = (e == hipSuccess) ? KF_OK : kf::fail_hip(e, "kf_server_create stream");
     for (Tick& k : s->fl)
-        if (!st && hipEventCreateWithFlags(&k.ev, hipEventDisableTiming) != hipSuccess)
+        if (!st && (hipEventCreateWithFlags(&k.ev, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&k.copied, hipEventDisableTiming) != hipSuccess))
             st = kf::fail(KF_ERR_HIP, "kf_server_create events");
     if (!st) st = kf_batch_create(&cfg->batch, reinterpret_cast<kf_stream_t>(s->s), &s->b);
     if (st) {
@@ -252,9 +308,9 @@ int kf_server_open(kf_server* s, const kf_session_config* c, int* ticket)
     int st = kf_session_bytes(c, &q.bytes);
     if (st) return st;
     // charged against the budget: what the batch holds for the session and the
-    // session's device frame buffer
+    // session's device frame buffers (one per tick slot)
     q.frame_bytes = sizeof(uint16_t) * (size_t)c->frame_width * c->frame_height;
-    q.bytes += q.frame_bytes;
+    q.bytes += q.frame_bytes * kInFlight;
     if (q.bytes > s->cfg.hbm_budget_bytes)
         return kf::fail(KF_ERR_BUDGET, "kf_server_open: the session alone exceeds the HBM budget");
     q.cap = s->cfg.queue_depth;
@@ -305,22 +361,14 @@ int kf_server_submit(kf_server* s, int id, const uint16_t* frame, double arrival
 int kf_server_poll(kf_server* s, int* launched)
 {
     if (!s) return kf::fail(KF_ERR_ARG, "kf_server_poll: null server");
-    if (launched) *launched = 0;
-    int st = KF_OK;
-    for (Tick& k : s->fl)
-        if ((st = retire(s, k, false))) return st;
-    for (Ticket& q : s->t)
-        if (q.state == 1 && q.closing && q.count == 0 && q.inflight == 0)
-            if ((st = release(s, q))) return st;
-    if ((st = admit(s))) return st;
-    return launch(s, launched);
+    return loop_turn(s, launched, false);
 }
 
 int kf_server_drain(kf_server* s)
 {
     if (!s) return kf::fail(KF_ERR_ARG, "kf_server_drain: null server");
     for (;;) {
-        int st = kf_server_poll(s, nullptr);
+        int st = loop_turn(s, nullptr, true);
         if (st) return st;
         bool busy = false, admitted_work = false, waiting_work = false;
         for (const Tick& k : s->fl) busy = busy || k.busy;

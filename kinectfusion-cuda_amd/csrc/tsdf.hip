@@ -115,9 +115,40 @@ __global__ __launch_bounds__(256) void depth_max_kernel(const float* __restrict_
 // for the depth test, so a brick costs one memory round trip, not two).  A
 // voxel whose depth test fails is read but not written.  Returns the number of
 // updated voxels.
+// x / b rounded to nearest from y = RN(1/b): q = RN(x*y), r = x - b*q (exact by
+// the fused multiply-add), RN(q + r*y) is RN(x/b) for every x and b whose
+// quotient and residual neither overflow nor underflow (Markstein's theorem;
+// checked exhaustively on the CPU for every x in [-mue, mue] at several mue
+// and for every divisor 1..129 of the count update).  The fuse's operands
+// (|x| >= 2^-25 or 0, b in [1e-4, 129]) stay far from both ranges.
+__device__ __forceinline__ float div_by_exact_reciprocal(float x, float b, float y)
+{
+    const float q = x * y;
+    const float r = fmaf(-q, b, x);
+    return fmaf(r, y, q);
+}
+
+// True when x may lie within a few ulp of an integer (0 and the image size
+// included): x*rcp(z) is within 2^-22 |x/z| of the rounded quotient x/z
+// (v_rcp_f32 is within 1 ulp), so outside this band both have the same sign,
+// the same integer part and the same side of every integer bound.
+__device__ __forceinline__ bool near_integer(float x)
+{
+    return fabsf(x - rintf(x)) <= fmaf(fabsf(x), 0x1p-20f, 0x1p-40f);
+}
+
+// The 8 voxels (one per z-slice kl) of one lane's column of a brick.
+// Projection (tsdf.cu:34-45): pe = inv_pose * (x, y, z, 1), pixel = K pe / pe.z;
+// only the pixel's integer part and the range test use the quotient, so it is
+// formed from the hardware reciprocal and recomputed by the correctly rounded
+// division only near an integer (the decisions are the exact quotient's).
+// Update (tsdf.cu:47-69): f = min(1, diff / mue) from the exact reciprocal of
+// mue; the precision form divides by the observation count + 1 with the exact
+// reciprocal of 1..129 from inv_count (LDS); every value written is the one the
+// plain quotients give.
 template <int FMT>
 __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v3 col, const float* zk, int64_t base,
-                                           float& fmin)
+                                           float& fmin, const float* inv_count)
 {
     typedef typename std::conditional<FMT == KF_VOXEL_F32, float, int16_t>::type FT;
     typedef typename std::conditional<FMT == KF_VOXEL_F32, float, uint8_t>::type WT;
@@ -128,6 +159,7 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
     WT wv[8];
     int pix[8];
     bool hit[8];
+    const float fw = (float)a.w, fh = (float)a.h;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
         const float z = zk[kl];
@@ -137,12 +169,24 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         bool ok = pe.z >= kMinDistance && pe.z < kMaxDistance;
         int q = 0;
         if (ok) {
-            // K pe; for a pinhole K (a.cull) the zero entries add exact zeros, so
-            // (fx x + cx z, fy y + cy z, z) is the same product
-            const v3 pv = a.cull ? mk3(a.K[0] * pe.x + a.K[6] * pe.z, a.K[4] * pe.y + a.K[7] * pe.z, pe.z)
-                                 : m3v(a.K, pe);
-            float px = pv.x / pv.z, py = pv.y / pv.z;
-            ok = px >= 0.0f && px < (float)a.w && py >= 0.0f && py < (float)a.h;
+            float px, py;
+            if (a.cull) {
+                // K pe for a pinhole K: the zero entries add exact zeros, so the product is
+                // (fx x + cx z, fy y + cy z, z)
+                const float vx = a.K[0] * pe.x + a.K[6] * pe.z, vy = a.K[4] * pe.y + a.K[7] * pe.z;
+                const float r = __builtin_amdgcn_rcpf(pe.z);
+                px = vx * r;
+                py = vy * r;
+                if (near_integer(px) || near_integer(py)) {
+                    px = vx / pe.z;
+                    py = vy / pe.z;
+                }
+            } else {
+                const v3 pv = m3v(a.K, pe);
+                px = pv.x / pv.z;
+                py = pv.y / pv.z;
+            }
+            ok = px >= 0.0f && px < fw && py >= 0.0f && py < fh;
             if (ok) q = (int)py * a.w + (int)px;
         }
         hit[kl] = ok;
@@ -170,7 +214,7 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         // division (most updated voxels lie in front of the surface) is skipped there
         float f = 1.0f;
         if (diff < a.mue) {
-            f = diff / a.mue;
+            f = div_by_exact_reciprocal(diff, a.mue, a.inv_mue);
             f = (f < 1.0f) ? f : 1.0f;
         }
         const int64_t q = base + 64 * kl;
@@ -184,17 +228,26 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
             fmin = fminf(fmin, Fv);
         } else {
             // precision form: unit weight per observation, count capped at 128
-            float Wv = (float)wv[kl];
+            const int wc = (int)wv[kl];
+            const float Wv = (float)wc;
             float Fv = snorm_decode((int16_t)fv[kl]);
-            Fv = (Fv * Wv + f) / (Wv + 1.0f);
+            Fv = div_by_exact_reciprocal(Fv * Wv + f, Wv + 1.0f, inv_count[wc + 1]);
             const int16_t qf = snorm_encode(Fv);
             F[q] = (FT)qf;
-            Wt[q] = (WT)(Wv < 128.0f ? Wv + 1.0f : 128.0f);
+            Wt[q] = (WT)(wc < 128 ? wc + 1 : 128);
             fmin = fminf(fmin, snorm_decode(qf));
         }
         ++cnt;
     }
     return cnt;
+}
+
+// Exact reciprocals 1/k, k = 0..129 (entry 0 unused), for the count update.
+constexpr int kInvCount = 130;
+__device__ __forceinline__ void fill_inv_count(float* t)
+{
+    for (int k = threadIdx.x; k < kInvCount; k += blockDim.x) t[k] = k ? 1.0f / (float)k : 0.0f;
+    __syncthreads();
 }
 
 __device__ __forceinline__ void load_inv_pose(const FuseArgs& a, float* ip)
@@ -257,7 +310,8 @@ typedef __attribute__((address_space(4))) const int ConstInt;
 typedef __attribute__((address_space(4))) const unsigned int ConstUint;
 
 template <int FMT>
-__device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, int64_t brick, int lane)
+__device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, int64_t brick, int lane,
+                                          const float* inv_count)
 {
     const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
     const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
@@ -269,7 +323,7 @@ __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, in
     const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
     const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
     float fmin = __int_as_float(0x7f800000);
-    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane, fmin);
+    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane, fmin, inv_count);
     // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
     fmin = wave_min(fmin);
     if (lane == 0 && fmin < bmin) {
@@ -306,6 +360,8 @@ __global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs 
                                                                     const unsigned int* count)
 {
     __shared__ int wave_cnt[kFuseWaves];
+    __shared__ float inv_count[kInvCount];
+    if (FMT != KF_VOXEL_F32) fill_inv_count(inv_count);
     // constant address space: no kernel writes the PoseState, the list or its
     // length while this one runs
     typedef __attribute__((address_space(4))) const PoseState ConstState;
@@ -331,7 +387,7 @@ __global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs 
     while (item < total) {
         const unsigned int next = item + waves;
         const int next_brick = next < cap ? ((ConstInt*)list)[next] : 0;
-        cnt += fuse_brick<FMT>(a, ip, brick, lane);
+        cnt += fuse_brick<FMT>(a, ip, brick, lane, inv_count);
         item = next;
         brick = next_brick;
     }
@@ -406,6 +462,7 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
     a.half = j.a.half;
     a.res = j.a.res;
     a.mue = j.a.mue;
+    a.inv_mue = j.a.inv_mue;
     a.zk = j.a.zk;
     a.f = j.a.f;
     a.wt = j.a.wt;
@@ -425,6 +482,8 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
 __global__ __launch_bounds__(64 * kFuseWaves, 8) void fuse_batch_list_kernel(const BatchHeader* hd, const FuseJob* jobs)
 {
     __shared__ int wave_cnt[kFuseWaves];
+    __shared__ float inv_count[kInvCount];
+    fill_inv_count(inv_count);
     // the tick's table, the list and its length are never written while the
     // kernel runs: read them with scalar loads through the constant address space
     typedef __attribute__((address_space(4))) const FuseJob ConstJob;
@@ -458,8 +517,8 @@ __global__ __launch_bounds__(64 * kFuseWaves, 8) void fuse_batch_list_kernel(con
 #pragma unroll
             for (int q = 0; q < 16; ++q) ip[q] = cs->inv_pose[q];
         }
-        if (a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(a, ip, brick, lane);
-        else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(a, ip, brick, lane);
+        if (a.format == KF_VOXEL_F32) cnt += fuse_brick<KF_VOXEL_F32>(a, ip, brick, lane, inv_count);
+        else cnt += fuse_brick<KF_VOXEL_SNORM16_U8>(a, ip, brick, lane, inv_count);
         e = ne;
         ++item;
     }
@@ -635,6 +694,7 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
     a.half = -v->width * 0.5f;
     a.res = v->res;
     a.mue = v->mue;
+    a.inv_mue = 1.0f / v->mue;
     a.zk = v->zk;
     a.f = v->f;
     a.wt = v->w;

@@ -652,7 +652,7 @@ class Server:
     continuous batching over a Batch, per-frame latency."""
 
     def __init__(self, max_sessions, hbm_budget_bytes, icp=None, use_graph=True, autotune=False, shape=None,
-                 queue_depth=8):
+                 queue_depth=8, launch_fill=0.9, max_wait_ms=2.0):
         init()
         c = kf_server_config()
         c.batch.max_sessions = max_sessions
@@ -664,6 +664,8 @@ class Server:
                 setattr(c.batch.shape, k, int(v))
         c.hbm_budget_bytes = int(hbm_budget_bytes)
         c.queue_depth = queue_depth
+        c.launch_fill = launch_fill
+        c.max_wait_ms = max_wait_ms
         self.cfg = c
         h = C.c_void_p()
         check(lib().kf_server_create(C.byref(c), C.byref(h)), "kf_server_create")

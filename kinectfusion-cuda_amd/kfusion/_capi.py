@@ -117,6 +117,8 @@ class kf_server_config(C.Structure):
         ("batch", kf_batch_config),
         ("hbm_budget_bytes", C.c_size_t),
         ("queue_depth", C.c_int),
+        ("launch_fill", C.c_float),
+        ("max_wait_ms", C.c_float),
     ]
 
 

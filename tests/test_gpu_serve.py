@@ -97,8 +97,8 @@ def test_alone_equals_mix_equals_pipeline(kf, sessions):
 def test_admission_waits_for_budget(kf, sessions):
     """A budget that holds two of the three sessions: the third waits until one closes,
     then runs to the same pose as the pipeline."""
-    # the server charges kf_session_bytes plus the session's device frame buffer
-    b = [kf.session_bytes(sessions[k][2]) + 2 * SPECS[k]["w"] * SPECS[k]["h"] for k in range(3)]
+    # the server charges kf_session_bytes plus the session's two device frame buffers
+    b = [kf.session_bytes(sessions[k][2]) + 2 * 2 * SPECS[k]["w"] * SPECS[k]["h"] for k in range(3)]
     budget = b[0] + b[2] + 1  # sessions 0 and 2 fit; 1 must wait for one of them to close
     assert budget < b[0] + b[1] + b[2]
     res, stats, lat = serve(kf, [0, 2, 1], sessions, budget=budget, keep_open=False, opens=[0.0, 0.0, 0.02])
