@@ -371,7 +371,7 @@ typedef struct kf_server_config {
                                     plus its two device frame buffers (2 x 2 bytes per pixel) */
     int queue_depth;             /* frames buffered per session (pinned host memory); 0 = 8 */
     float launch_fill;           /* a tick is launched once frames of at least this fraction of the resident
-                                    sessions wait; 0 = 0.9 */
+                                    sessions wait; 0 = 0.5 */
     float max_wait_ms;           /* ...or, with no tick running, once the oldest waiting frame has waited this
                                     long; 0 = 2 ms, < 0 = never wait (launch whenever the GPU is idle) */
 } kf_server_config;

@@ -281,7 +281,7 @@ int kf_server_create(const kf_server_config* cfg, kf_server** out)
     kf_server* s = new kf_server();
     s->cfg = *cfg;
     if (s->cfg.queue_depth <= 0) s->cfg.queue_depth = 8;
-    if (!(s->cfg.launch_fill > 0.0f)) s->cfg.launch_fill = 0.9f;
+    if (!(s->cfg.launch_fill > 0.0f)) s->cfg.launch_fill = 0.5f;
     if (s->cfg.launch_fill > 1.0f) s->cfg.launch_fill = 1.0f;
     if (s->cfg.max_wait_ms == 0.0f) s->cfg.max_wait_ms = 2.0f;
     hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);

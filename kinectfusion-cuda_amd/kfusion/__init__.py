@@ -652,7 +652,7 @@ class Server:
     continuous batching over a Batch, per-frame latency."""
 
     def __init__(self, max_sessions, hbm_budget_bytes, icp=None, use_graph=True, autotune=False, shape=None,
-                 queue_depth=8, launch_fill=0.9, max_wait_ms=2.0):
+                 queue_depth=8, launch_fill=0.5, max_wait_ms=2.0):
         init()
         c = kf_server_config()
         c.batch.max_sessions = max_sessions
