@@ -254,33 +254,44 @@ __device__ __forceinline__ int wave_sum(int v)
 
 // ---------------------------------------------------------------------------
 // TSDF volume layout in HBM: 8x8x8 bricks of 512 voxels, bricks ordered x then
-// y then z, voxels inside a brick ordered x then y then z.  A brick is 2 KiB of
-// FP32 (one plane), so a wave's 64 lanes touch whole 128-byte lines when they
-// sweep a brick, and a trilinear stencil usually stays inside one brick.
+// y then z, voxels inside a brick ordered z then x then y (z fastest).  The 8
+// voxels of a brick column (i, j, 0..7) are contiguous, so the fusion sweep
+// (one lane per column) reads and writes a column with one or two wide accesses
+// and a wave covers a brick's planes contiguously; a trilinear stencil usually
+// stays inside one brick.
 constexpr int kBrick = 8;
 constexpr int kBrickVoxels = 512;
 
 __device__ __forceinline__ int64_t brick_index(int nb, int i, int j, int k)
 {
     int64_t b = ((int64_t)(k >> 3) * nb + (j >> 3)) * nb + (i >> 3);
-    return b * kBrickVoxels + (((k & 7) * 8 + (j & 7)) * 8 + (i & 7));
+    return b * kBrickVoxels + (((j & 7) * 8 + (i & 7)) * 8 + (k & 7));
 }
 // The same index as a sum of per-axis terms, brick_index = ox(i) + oy(j) + oz(k),
 // in 32 bits (volumes are at most 1024^3 voxels): the 8 corners of a trilinear
 // cell then cost 6 terms and 12 additions, and a load is a 32-bit offset from
 // the plane's base.
-__device__ __forceinline__ uint32_t brick_ox(int i) { return (uint32_t)(i >> 3) * 512u + (uint32_t)(i & 7); }
+__device__ __forceinline__ uint32_t brick_ox(int i) { return (uint32_t)(i >> 3) * 512u + (uint32_t)(i & 7) * 8u; }
 __device__ __forceinline__ uint32_t brick_oy(int nb, int j)
 {
-    return (uint32_t)(j >> 3) * (uint32_t)nb * 512u + (uint32_t)(j & 7) * 8u;
+    return (uint32_t)(j >> 3) * (uint32_t)nb * 512u + (uint32_t)(j & 7) * 64u;
 }
 __device__ __forceinline__ uint32_t brick_oz(int nb, int k)
 {
-    return (uint32_t)(k >> 3) * (uint32_t)nb * (uint32_t)nb * 512u + (uint32_t)(k & 7) * 64u;
+    return (uint32_t)(k >> 3) * (uint32_t)nb * (uint32_t)nb * 512u + (uint32_t)(k & 7);
 }
 
-// Super-bricks: 8x8x8 bricks (64^3 voxels) carry a lower bound of F too, kept
-// as an ordered integer key so it can be lowered with an integer atomicMin.
+// Brick bounds (raycast skipping): brick (bi, bj, bk) keeps a lower bound of F
+// over its halo, the voxels [8bi, 8bi + 8] x [8bj, 8bj + 8] x [8bk, 8bk + 8]
+// clamped to the volume: its own 512 voxels and the first voxel plane of its
+// +x, +y and +z neighbours.  A trilinear stencil spans (i0, i0 + 1) per axis
+// (clamped), so the bound of the brick holding its first corner (i0, j0, k0)
+// covers all 8 corners, also where the stencil crosses into the next brick.
+// Several waves lower one bound in a fuse (the brick's own and those of the
+// bricks whose halo reaches into it), so it is kept as an ordered integer key
+// (order_key) and lowered with an integer atomicMin.
+// Super-bricks: 8x8x8 bricks (64^3 voxels) carry a lower bound of F over their
+// own voxels too, in the same key form.
 constexpr int kSuper = 64;
 __device__ __forceinline__ int order_key(float f)
 {

@@ -78,7 +78,7 @@ struct kf_volume {
     void* w = nullptr;        // float* or uint8_t* plane
     float* zk = nullptr;      // z coordinate of each k slice (repeated addition, tsdf.cu:27-31)
     unsigned long long* touched = nullptr;  // per-workgroup counts of fused voxels (summed on read)
-    float* brick_min = nullptr;             // per-brick lower bound of F (raycast skipping)
+    int* brick_key = nullptr;               // per-brick halo lower bound of F, order_key form (raycast skipping)
     int* super_key = nullptr;               // per-super-brick lower bound of F (order_key)
     int ns = 0;                             // super-bricks per axis (ceil(nb / 8))
     int* brick_list = nullptr;              // visible bricks of the current fuse

@@ -69,7 +69,7 @@ struct FuseArgs {
     void* f;
     void* wt;
     unsigned long long* touched;
-    float* brick_min;
+    int* brick_key;
     int* super_key;
     int ns;
     int cull;     // 1 when K is a plain pinhole matrix (culling planes valid)
@@ -109,7 +109,7 @@ int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, 
 // ---- raycast (raycast.hip)
 struct RaycastVolume {
     const void* f;
-    const float* brick_min;  // per-brick lower bound of F (null: no skipping)
+    const int* brick_key;    // per-brick halo lower bound of F, order_key form (null: no skipping)
     const int* super_key;    // per-super-brick lower bound of F, order_key form
     int n, nb, ns, format;
     float width, mue;

@@ -22,11 +22,18 @@ namespace kf {
 namespace {
 
 constexpr int kMaxMarchSteps = 1 << 20;
-// A sample whose 8 trilinear corners lie in one brick whose lower bound of F is
-// at least this value is >= 0.99 (convex weights; 1e-3 of margin covers the
-// rounding of the three lerp levels), so it can neither end the march nor
-// shorten the step.
+// Skipping: a brick's bound covers every stencil whose first corner lies in it
+// (its halo, kf_device.hpp), a super-brick's the stencils inside it.  On a
+// large step a sample whose bound is at least kSkipBound is >= 0.99 (convex
+// weights; 1e-3 of margin covers the rounding of the three lerp levels), so
+// it can neither end the march nor shorten the step.  On a small step only a
+// negative sample changes the march (the step is already small), and a sample
+// whose bound is >= 0 is >= 0 (non-negative corners and weights give a
+// non-negative value in any rounding), so kSmallSkipBound = 0 applies there.
+// A skipped sample only sets prev_t; its value is computed if and when the
+// zero-crossing formula needs it.
 constexpr float kSkipBound = 0.991f;
+constexpr float kSmallSkipBound = 0.0f;
 
 
 // The fast skip test works on texel coordinates computed with a reciprocal
@@ -161,24 +168,34 @@ __device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float
     return trilinear<FMT>(F, nb, stencil(n, u, v, s));
 }
 
-// True when the sample is known to be >= 0.99 without reading the voxels.
-__device__ __forceinline__ bool skippable(const RaycastVolume& a, const Stencil& t)
+// Brick holding the stencil's first corner (its halo bound covers the stencil).
+__device__ __forceinline__ int stencil_brick(const RaycastVolume& a, const Stencil& t)
 {
-    if (!a.brick_min) return false;
-    const int b0 = ((t.k0 >> 3) * a.nb + (t.j0 >> 3)) * a.nb + (t.i0 >> 3);
-    const int b1 = ((t.k1 >> 3) * a.nb + (t.j1 >> 3)) * a.nb + (t.i1 >> 3);
-    return b0 == b1 && a.brick_min[b0] >= kSkipBound;
+    return ((t.k0 >> 3) * a.nb + (t.j0 >> 3)) * a.nb + (t.i0 >> 3);
+}
+
+// True when the sample's bound is >= thr (the exact stencil), through the
+// brick cache (ck, cbk).
+__device__ __forceinline__ bool skippable(const RaycastVolume& a, const Stencil& t, float thr, int& ck, float& cbk)
+{
+    if (!a.brick_key) return false;
+    const int b0 = stencil_brick(a, t);
+    if (b0 != ck) {
+        ck = b0;
+        cbk = order_value(a.brick_key[b0]);
+    }
+    return cbk >= thr;
 }
 
 // Conservative skip test on approximate texel coordinates (xa, ya, za).
-// Returns 1 when the exact stencil lies inside one super-brick (or one brick)
-// whose lower bound of F is >= kSkipBound, 0 when it lies inside one brick
-// whose bound is below it (so skippable() would say no), -1 when the margins
-// leave it open.  The last super-brick's and brick's bounds are cached in
-// (cs, cb) and (ck, cbk), so a march through free space, or along a brick, reads
-// no memory.
-__device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float ya, float za, int& cs, float& cb,
-                                         int& ck, float& cbk)
+// Returns 1 when the exact stencil lies inside one super-brick, or has its
+// first corner in one brick, whose bound of F is >= thr, 0 when its first
+// corner is in one brick whose bound is below it (so skippable() would say
+// no), -1 when the margins leave it open.  The last super-brick's and brick's
+// bounds are cached in (cs, cb) and (ck, cbk), so a march through free space,
+// or along a brick, reads no memory.
+__device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float ya, float za, float thr, int& cs,
+                                         float& cb, int& ck, float& cbk)
 {
     const float hi = (float)a.n - 1.0f - kMargin;
     if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return -1;
@@ -192,55 +209,72 @@ __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float
             cs = sidx;
             cb = order_value(a.super_key[sidx]);
         }
-        if (cb >= kSkipBound) return 1;
+        if (cb >= thr) return 1;
     }
+    // the exact first corner is in brick (bx, by, bz) when the coordinate is
+    // kMargin inside the brick's [0, 8) span on every axis
     const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
     const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
-    const float bhi = (float)(kBrick - 1) - kMargin;
+    const float bhi = (float)kBrick - kMargin;
     if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi) {
         const int bidx = (bz * a.nb + by) * a.nb + bx;
         if (bidx != ck) {
             ck = bidx;
-            cbk = a.brick_min[bidx];
+            cbk = order_value(a.brick_key[bidx]);
         }
-        return cbk >= kSkipBound ? 1 : 0;
+        return cbk >= thr ? 1 : 0;
     }
     return -1;
 }
 
-// Samples after the one at t (same step, t + k*step by repeated addition) that
-// are certainly skipped through the cached super-brick bound.  The march's
-// approximate texel coordinates are linear in t up to rounding: t stays below
-// 6 m, so each addition is off by at most 2^-22 m, and k of them by
-// k * 2^-22 m * (texels per metre); k is capped so that this stays below
-// kMargin / 2, the coordinates' own rounding is a few 1e-4 texel.  A sample
-// whose linear coordinate is 2 kMargin inside the cached super-brick (and the
-// volume) thus has a computed coordinate kMargin inside it, and fast_skip
-// would return 1 for it from the cache.  0 when the bound does not allow
-// skipping or the current sample is near the super-brick's faces.
-__device__ __forceinline__ int super_run(const RaycastVolume& a, v3 o, v3 dir, float t, float step, float invw,
-                                         float nf, int cs, float cb)
+// Samples after the one at t (same step, t + k*step by repeated addition)
+// whose stencils certainly stay in the texel region [lo, hi] per axis (the
+// first corner at most hi - 1).  The march's approximate texel coordinates are
+// linear in t up to rounding: t stays below 6 m, so each addition is off by at
+// most 2^-22 m, and k of them by k * 2^-22 m * (texels per metre); k is capped
+// so that this stays below kMargin / 2, the coordinates' own rounding is a few
+// 1e-4 texel.  A sample whose linear coordinate is 2 kMargin inside the region
+// thus has a computed coordinate kMargin inside it, and fast_skip's tests hold
+// for it.  The step count is formed with the hardware reciprocal: a relative
+// error of a few 1e-7 on at most 64 texels of distance moves the last counted
+// sample by under 1e-4 texel, far inside the kMargin / 2 left for it.  -1 when
+// the sample at t itself is not 2 kMargin inside.
+__device__ __forceinline__ int region_run(v3 o, v3 dir, float t, float step, float invw, float nf, const int* lo,
+                                          const int* hi)
 {
-    if (cs < 0 || !(cb >= kSkipBound)) return 0;
     const v3 p = add3(o, mul3s(dir, t));
-    const float xa = approx_texel(p.x, invw, nf), ya = approx_texel(p.y, invw, nf),
-                za = approx_texel(p.z, invw, nf);
-    const int sx = cs % a.ns, sy = (cs / a.ns) % a.ns, sz = cs / (a.ns * a.ns);
-    const float m2 = 2.0f * kMargin, top = (float)a.n - 1.0f;
-    float kmax = 20000.0f / (nf * invw);  // k * 2^-22 * texels per metre <= kMargin / 2
-    const float pos[3] = {xa, ya, za};
+    const float pos[3] = {approx_texel(p.x, invw, nf), approx_texel(p.y, invw, nf), approx_texel(p.z, invw, nf)};
     const float d[3] = {dir.x, dir.y, dir.z};
-    const int sc[3] = {sx, sy, sz};
+    const float m2 = 2.0f * kMargin;
+    float kmax = 20000.0f / (nf * invw);  // k * 2^-22 * texels per metre <= kMargin / 2
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const float lo = (float)(sc[ax] * kSuper) + m2;
-        const float hi = fminf((float)(sc[ax] * kSuper + kSuper - 1), top) - m2;
-        if (!(pos[ax] >= lo && pos[ax] <= hi)) return 0;
+        const float l = (float)lo[ax] + m2, h = (float)hi[ax] - m2;
+        if (!(pos[ax] >= l && pos[ax] <= h)) return -1;
         const float rate = d[ax] * invw * nf * step;  // texels per step
-        if (rate > 0.0f) kmax = fminf(kmax, (hi - pos[ax]) / rate);
-        else if (rate < 0.0f) kmax = fminf(kmax, (lo - pos[ax]) / rate);
+        const float inv = __builtin_amdgcn_rcpf(rate);
+        if (rate > 0.0f) kmax = fminf(kmax, (h - pos[ax]) * inv);
+        else if (rate < 0.0f) kmax = fminf(kmax, (l - pos[ax]) * inv);
     }
-    return kmax >= 1.0f ? (int)kmax : 0;
+    return (int)kmax;
+}
+
+// Samples after the one at t inside the cached super-brick, which covers
+// stencils inside it; 0 when its bound is below thr or the sample at t is
+// near its faces.
+__device__ __forceinline__ int super_run(const RaycastVolume& a, v3 o, v3 dir, float t, float step, float invw,
+                                         float nf, float thr, int cs, float cb)
+{
+    if (cs < 0 || !(cb >= thr)) return 0;
+    const int sc[3] = {cs % a.ns, (cs / a.ns) % a.ns, cs / (a.ns * a.ns)};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        lo[ax] = sc[ax] * kSuper;
+        hi[ax] = min(sc[ax] * kSuper + kSuper - 1, a.n - 1);
+    }
+    const int k = region_run(o, dir, t, step, invw, nf, lo, hi);
+    return k > 0 ? k : 0;
 }
 
 // The corner values of the last trilinear cell a march read: consecutive small
@@ -270,10 +304,10 @@ __device__ __forceinline__ float trilinear_cached(const void* F, int nb, const S
     return (1.0f - c) * g0 + c * g1;
 }
 
-// A lane marches at most kLaneBudget samples of its ray unless at most
-// kWorkFloor of them needed voxel reads (a ray crossing free space to a distant
-// surface is cheap and stays); a longer ray (about 1 in 40 grazes a surface and
-// takes hundreds of small steps) is appended to a list that a second pass
+// A lane marches at most kLaneBudget samples of its ray unless it took at most
+// kWorkFloor trips of the march loop for them (a ray crossing free space skips
+// many samples per trip and stays); a longer ray (about 1 in 40 grazes a
+// surface and takes hundreds of small steps) is appended to a list that a second pass
 // finishes with a whole wave per ray, so no wave of the first pass waits on
 // one long ray.
 // The single-session first pass (its waves' tail bounds the frame) hands rays
@@ -282,20 +316,21 @@ constexpr int kLaneBudget = 40;
 constexpr int kBatchLaneBudget = 48;
 constexpr int kWorkFloor = 16;
 
-// One trilinear evaluation of the march at t: 1 = skipped (the sample is known
-// to be >= 0.99), 0 = evaluated into f.
+// One trilinear evaluation of the march at t: 1 = skipped (the sample's bound
+// is >= thr: kSkipBound on a large step, kSmallSkipBound on a small one),
+// 0 = evaluated into f.
 template <int FMT>
 __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir, float t, float width, float invw,
-                                            float nf, int& cs, float& cb, int& ck, float& cbk, CellCache& cc,
-                                            float& f)
+                                            float nf, float thr, int& cs, float& cb, int& ck, float& cbk,
+                                            CellCache& cc, float& f)
 {
     const v3 p = add3(o, mul3s(dir, t));
-    const int fs = a.brick_min ? fast_skip(a, approx_texel(p.x, invw, nf), approx_texel(p.y, invw, nf),
-                                           approx_texel(p.z, invw, nf), cs, cb, ck, cbk)
+    const int fs = a.brick_key ? fast_skip(a, approx_texel(p.x, invw, nf), approx_texel(p.y, invw, nf),
+                                           approx_texel(p.z, invw, nf), thr, cs, cb, ck, cbk)
                                : 0;
     if (fs == 1) return 1;
     const Stencil st = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
-    if (fs < 0 && skippable(a, st)) return 1;
+    if (fs < 0 && skippable(a, st, thr, ck, cbk)) return 1;
     f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
     return 0;
 }
@@ -309,9 +344,9 @@ __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir
 constexpr int kGroup = 2;
 
 // fast_skip without memory access: 1 when the cached super-brick or brick bound
-// shows the exact stencil is >= 0.99, else 0 (evaluate the sample).
-__device__ __forceinline__ int cached_skip(const RaycastVolume& a, float xa, float ya, float za, int cs, float cb,
-                                           int ck, float cbk)
+// is >= thr for the exact stencil, else 0 (evaluate the sample).
+__device__ __forceinline__ int cached_skip(const RaycastVolume& a, float xa, float ya, float za, float thr, int cs,
+                                           float cb, int ck, float cbk)
 {
     const float hi = (float)a.n - 1.0f - kMargin;
     if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return 0;
@@ -320,13 +355,13 @@ __device__ __forceinline__ int cached_skip(const RaycastVolume& a, float xa, flo
     const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
     const float shi = (float)(kSuper - 1) - kMargin;
     if (ux >= kMargin && uy >= kMargin && uz >= kMargin && ux < shi && uy < shi && uz < shi &&
-        (sz * a.ns + sy) * a.ns + sx == cs && cb >= kSkipBound)
+        (sz * a.ns + sy) * a.ns + sx == cs && cb >= thr)
         return 1;
     const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
     const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
-    const float bhi = (float)(kBrick - 1) - kMargin;
+    const float bhi = (float)kBrick - kMargin;
     if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi &&
-        (bz * a.nb + by) * a.nb + bx == ck && cbk >= kSkipBound)
+        (bz * a.nb + by) * a.nb + bx == ck && cbk >= thr)
         return 1;
     return 0;
 }
@@ -468,7 +503,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     const float invw = 1.0f / width, nf = (float)a.n;
     float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f, t = nearv;
     bool prev_skipped = false, done = false, can_defer = true;
-    int work = 0;  // samples evaluated (memory read) so far
+    int work = 0;  // trips of the march loop so far (each a skip test or an evaluation)
     int cs = -1, ck = -1;
     float cb = 0.0f, cbk = 0.0f;
     CellCache cc;
@@ -477,7 +512,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     for (;;) {
         if (!(t < farv)) break;
         if (++guard > kMaxMarchSteps) break;
-        if (can_defer && guard > rq.budget && work > kWorkFloor) {
+        if (can_defer && ++work > kWorkFloor && guard > rq.budget) {
             const unsigned slot = atomicAdd(rq.count, 1u);
             if (slot < rq.cap) {
                 DeferredRay d;
@@ -496,10 +531,10 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         }
         if (step == large) {
             float f = 0.0f;
-            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, cs, cb, ck, cbk, cc, f)) {
+            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, kSkipBound, cs, cb, ck, cbk, cc, f)) {
                 // this sample and the following ones inside the same free super-brick are
                 // skipped: positions by the same additions, no coordinates computed
-                int run = super_run(a, o, dir, t, step, invw, nf, cs, cb);
+                int run = super_run(a, o, dir, t, step, invw, nf, kSkipBound, cs, cb);
                 prev_skipped = true;
                 prev_t = t;
                 t = t + step;
@@ -510,7 +545,6 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
                 }
                 continue;
             }
-            ++work;
             if (f < 0.0f) {
                 if (t == nearv) break;
                 t = t - mue;  // back off and march on with small steps
@@ -539,24 +573,19 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
 #pragma unroll
         for (int k = 0; k < kGroup; ++k) {
             const v3 p = add3(o, mul3s(dir, tq[k]));
-            const int sk = a.brick_min ? cached_skip(a, approx_texel(p.x, invw, nf),
+            const int sk = a.brick_key ? cached_skip(a, approx_texel(p.x, invw, nf),
                                                      approx_texel(p.y, invw, nf),
-                                                     approx_texel(p.z, invw, nf), cs, cb, ck, cbk)
+                                                     approx_texel(p.z, invw, nf), kSmallSkipBound, cs, cb, ck, cbk)
                                        : 0;
             st[k] = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
             bool skip = sk == 1;
-            if (!skip && a.brick_min) {
-                const Stencil& q = st[k];
-                const int b0 = ((q.k0 >> 3) * a.nb + (q.j0 >> 3)) * a.nb + (q.i0 >> 3);
-                const int b1 = ((q.k1 >> 3) * a.nb + (q.j1 >> 3)) * a.nb + (q.i1 >> 3);
-                skip = b0 == b1 && b0 == ck && cbk >= kSkipBound;
-            }
+            if (!skip && a.brick_key) skip = stencil_brick(a, st[k]) == ck && cbk >= kSmallSkipBound;
             ev[k] = ex[k] && !skip;
         }
         // the bounds of the last existing sample's brick and super-brick (for the
         // next group's tests; selects, no dynamic index), then the corner loads of
         // every evaluated sample
-        if (a.brick_min) {
+        if (a.brick_key) {
             int bidx = 0, sidx = 0;
 #pragma unroll
             for (int k = 0; k < kGroup; ++k) {
@@ -568,7 +597,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
             }
             if (bidx != ck) {
                 ck = bidx;
-                cbk = a.brick_min[bidx];
+                cbk = order_value(a.brick_key[bidx]);
             }
             if (sidx != cs) {
                 cs = sidx;
@@ -640,7 +669,6 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
                 prev_t = tq[k];
                 continue;
             }
-            ++work;
             const float ca = st[k].a, cbw = st[k].b, cc2 = st[k].c;
             const float e00 = (1.0f - ca) * cv[k][0] + ca * cv[k][1], e10 = (1.0f - ca) * cv[k][2] + ca * cv[k][3];
             const float e01 = (1.0f - ca) * cv[k][4] + ca * cv[k][5], e11 = (1.0f - ca) * cv[k][6] + ca * cv[k][7];
@@ -699,7 +727,9 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
         const bool end = !(tk < g.farv) || gb + lane > kMaxMarchSteps;
         float f = 0.0f;
         int sk = 1;
-        if (!end) sk = march_sample<FMT>(a, g.o, g.dir, tk, width, invw, nf, cs, cb, ck, cbk, cc, f);
+        if (!end)
+            sk = march_sample<FMT>(a, g.o, g.dir, tk, width, invw, nf, step == large ? kSkipBound : kSmallSkipBound,
+                                   cs, cb, ck, cbk, cc, f);
         const bool neg = !end && !sk && f < 0.0f;
         const bool slow = !end && !sk && step == large && f < 0.99f;
         const uint64_t evm = __ballot(end || neg || slow);
@@ -856,7 +886,7 @@ template <class CJ>
 __device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, RaycastLevel& L)
 {
     vol.f = j.vol.f;
-    vol.brick_min = j.vol.brick_min;
+    vol.brick_key = j.vol.brick_key;
     vol.super_key = j.vol.super_key;
     vol.n = j.vol.n;
     vol.nb = j.vol.nb;
@@ -968,7 +998,7 @@ RaycastVolume raycast_volume(const kf_volume* v)
 {
     RaycastVolume r{};
     r.f = v->f;
-    r.brick_min = v->brick_min;
+    r.brick_key = v->brick_key;
     r.super_key = v->super_key;
     r.n = v->n;
     r.nb = v->nb;

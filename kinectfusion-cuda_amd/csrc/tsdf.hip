@@ -5,9 +5,10 @@
 //   KF_VOXEL_F32        : f float, w float          (8 B / voxel)
 //   KF_VOXEL_SNORM16_U8 : f snorm16, w uint8 count  (3 B / voxel)
 // A cull pass lists the bricks whose bounding sphere meets the view frustum;
-// then one wave fuses one listed brick at a time, 8 voxels per lane (one per
-// z-slice), so every plane access of a wave is a contiguous 256 B (f32) run and
-// the listed bricks are spread evenly over the waves.  Bricks whose bounding sphere lies outside the view frustum
+// then one wave fuses one listed brick at a time, one brick column (8 voxels
+// along z, contiguous in the plane) per lane, so a wave reads and writes a
+// brick's planes as contiguous 16-byte-per-lane accesses, and the listed
+// bricks are spread evenly over the waves.  Bricks whose bounding sphere lies outside the view frustum
 // are skipped before any voxel is visited; the test is conservative, so the
 // result is identical to visiting every voxel.
 #include <stdio.h>
@@ -24,6 +25,14 @@
 namespace kf {
 
 namespace {
+
+// List entries name a brick by its coordinates, bi | bj << 10 | bk << 20
+// (volumes are at most 1024^3 voxels: 128 bricks per axis), so the sweep
+// forms its linear index with two multiply-adds instead of two divisions.
+__host__ __device__ __forceinline__ unsigned pack_brick(int bi, int bj, int bk)
+{
+    return (unsigned)bi | (unsigned)bj << 10 | (unsigned)bk << 20;
+}
 
 __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip, int bi, int bj, int bk)
 {
@@ -137,6 +146,81 @@ __device__ __forceinline__ bool near_integer(float x)
     return fabsf(x - rintf(x)) <= fmaf(fabsf(x), 0x1p-20f, 0x1p-40f);
 }
 
+// A brick column's planes in registers: the 8 voxels (i, j, 0..7) are
+// contiguous, so f and w move as one or two 16-byte accesses each (FP32: two
+// float4 per plane; snorm16: 8 codes in a uint4, 8 counts in a uint2).
+template <int FMT>
+struct ColumnPlanes;
+
+template <>
+struct ColumnPlanes<KF_VOXEL_F32> {
+    float f[8], w[8];
+    __device__ __forceinline__ void clear()
+    {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = w[k] = 0.0f;
+    }
+    __device__ __forceinline__ void load(const FuseArgs& a, int64_t base)
+    {
+        const float4* F = reinterpret_cast<const float4*>(static_cast<const float*>(a.f) + base);
+        const float4* W = reinterpret_cast<const float4*>(static_cast<const float*>(a.wt) + base);
+        const float4 f0 = F[0], f1 = F[1], w0 = W[0], w1 = W[1];
+        f[0] = f0.x, f[1] = f0.y, f[2] = f0.z, f[3] = f0.w, f[4] = f1.x, f[5] = f1.y, f[6] = f1.z, f[7] = f1.w;
+        w[0] = w0.x, w[1] = w0.y, w[2] = w0.z, w[3] = w0.w, w[4] = w1.x, w[5] = w1.y, w[6] = w1.z, w[7] = w1.w;
+    }
+    __device__ __forceinline__ void store(const FuseArgs& a, int64_t base) const
+    {
+        float4* F = reinterpret_cast<float4*>(static_cast<float*>(a.f) + base);
+        float4* W = reinterpret_cast<float4*>(static_cast<float*>(a.wt) + base);
+        F[0] = make_float4(f[0], f[1], f[2], f[3]);
+        F[1] = make_float4(f[4], f[5], f[6], f[7]);
+        W[0] = make_float4(w[0], w[1], w[2], w[3]);
+        W[1] = make_float4(w[4], w[5], w[6], w[7]);
+    }
+};
+
+template <>
+struct ColumnPlanes<KF_VOXEL_SNORM16_U8> {
+    uint4 f;  // codes 2m (low half) and 2m+1 (high half) in word m
+    uint2 w;  // count k in byte k % 4 of word k / 4
+    __device__ __forceinline__ void clear()
+    {
+        f = make_uint4(0u, 0u, 0u, 0u);
+        w = make_uint2(0u, 0u);
+    }
+    __device__ __forceinline__ void load(const FuseArgs& a, int64_t base)
+    {
+        f = *reinterpret_cast<const uint4*>(static_cast<const int16_t*>(a.f) + base);
+        w = *reinterpret_cast<const uint2*>(static_cast<const uint8_t*>(a.wt) + base);
+    }
+    __device__ __forceinline__ void store(const FuseArgs& a, int64_t base) const
+    {
+        *reinterpret_cast<uint4*>(static_cast<int16_t*>(a.f) + base) = f;
+        *reinterpret_cast<uint2*>(static_cast<uint8_t*>(a.wt) + base) = w;
+    }
+    __device__ __forceinline__ static uint32_t word(const uint4& v, int m)
+    {
+        return m == 0 ? v.x : (m == 1 ? v.y : (m == 2 ? v.z : v.w));
+    }
+    __device__ __forceinline__ int16_t code(int k) const
+    {
+        return (int16_t)(uint16_t)(word(f, k >> 1) >> (16 * (k & 1)));
+    }
+    __device__ __forceinline__ int count(int k) const
+    {
+        return (int)(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xffu);
+    }
+    __device__ __forceinline__ void set(int k, int16_t q, int c)
+    {
+        const uint32_t sh = 16u * (uint32_t)(k & 1), m = 0xffffu << sh, v = (uint32_t)(uint16_t)q << sh;
+        uint32_t& fw = (k >> 1) == 0 ? f.x : ((k >> 1) == 1 ? f.y : ((k >> 1) == 2 ? f.z : f.w));
+        fw = (fw & ~m) | v;
+        const uint32_t bs = 8u * (uint32_t)(k & 3), bm = 0xffu << bs;
+        uint32_t& ww = k < 4 ? w.x : w.y;
+        ww = (ww & ~bm) | ((uint32_t)c << bs);
+    }
+};
+
 // The 8 voxels (one per z-slice kl) of one lane's column of a brick.
 // Projection (tsdf.cu:34-45): pe = inv_pose * (x, y, z, 1), pixel = K pe / pe.z;
 // only the pixel's integer part and the range test use the quotient, so it is
@@ -145,18 +229,16 @@ __device__ __forceinline__ bool near_integer(float x)
 // Update (tsdf.cu:47-69): f = min(1, diff / mue) from the exact reciprocal of
 // mue; the precision form divides by the observation count + 1 with the exact
 // reciprocal of 1..129 from inv_count (LDS); every value written is the one the
-// plain quotients give.
+// plain quotients give.  fmin takes the smallest value written, fk0 the value
+// written to voxel 0 (the brick's k = 0 plane).  The column's planes are read
+// when any of its voxels projects into the image and written back whole when
+// any was updated (the others keep the bits they were read with; one wave owns
+// the brick).
 template <int FMT>
 __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v3 col, const float* zk, int64_t base,
-                                           float& fmin, const float* inv_count)
+                                           float& fmin, float& fk0, const float* inv_count)
 {
-    typedef typename std::conditional<FMT == KF_VOXEL_F32, float, int16_t>::type FT;
-    typedef typename std::conditional<FMT == KF_VOXEL_F32, float, uint8_t>::type WT;
-    FT* F = static_cast<FT*>(a.f);
-    WT* Wt = static_cast<WT*>(a.wt);
     float dep[8], pz[8];
-    FT fv[8];
-    WT wv[8];
     int pix[8];
     bool hit[8];
     const float fw = (float)a.w, fh = (float)a.h;
@@ -193,17 +275,16 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
         pix[kl] = q;
         pz[kl] = pe.z;
     }
+    bool any = false;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
         dep[kl] = 0.0f;
-        fv[kl] = FT(0);
-        wv[kl] = WT(0);
-        if (hit[kl]) {
-            dep[kl] = a.depth[pix[kl]];
-            fv[kl] = F[base + 64 * kl];
-            wv[kl] = Wt[base + 64 * kl];
-        }
+        if (hit[kl]) dep[kl] = a.depth[pix[kl]];
+        any = any || hit[kl];
     }
+    ColumnPlanes<FMT> v;
+    v.clear();
+    if (any) v.load(a, base);
     int cnt = 0;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
@@ -217,28 +298,30 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
             f = div_by_exact_reciprocal(diff, a.mue, a.inv_mue);
             f = (f < 1.0f) ? f : 1.0f;
         }
-        const int64_t q = base + 64 * kl;
-        if (FMT == KF_VOXEL_F32) {
+        if constexpr (FMT == KF_VOXEL_F32) {
             const float wt = 1.0f / d;
-            float Fv = (float)fv[kl], Wv = (float)wv[kl];
+            float Fv = v.f[kl], Wv = v.w[kl];
             Fv = (Fv * Wv + f * wt) / (Wv + wt);
             Wv = Wv + wt;
-            F[q] = (FT)Fv;
-            Wt[q] = (WT)Wv;
+            v.f[kl] = Fv;
+            v.w[kl] = Wv;
             fmin = fminf(fmin, Fv);
+            if (kl == 0) fk0 = Fv;
         } else {
             // precision form: unit weight per observation, count capped at 128
-            const int wc = (int)wv[kl];
+            const int wc = v.count(kl);
             const float Wv = (float)wc;
-            float Fv = snorm_decode((int16_t)fv[kl]);
+            float Fv = snorm_decode(v.code(kl));
             Fv = div_by_exact_reciprocal(Fv * Wv + f, Wv + 1.0f, inv_count[wc + 1]);
             const int16_t qf = snorm_encode(Fv);
-            F[q] = (FT)qf;
-            Wt[q] = (WT)(wc < 128 ? wc + 1 : 128);
-            fmin = fminf(fmin, snorm_decode(qf));
+            v.set(kl, qf, wc < 128 ? wc + 1 : 128);
+            const float written = snorm_decode(qf);
+            fmin = fminf(fmin, written);
+            if (kl == 0) fk0 = written;
         }
         ++cnt;
     }
+    if (cnt) v.store(a, base);
     return cnt;
 }
 
@@ -272,10 +355,13 @@ __global__ __launch_bounds__(kCullThreads) void fuse_cull_kernel(FuseArgs a, int
     if (blockIdx.x == 0 && threadIdx.x == 0) count[1 - parity] = 0u;
     const int64_t brick = (int64_t)blockIdx.x * kCullThreads + threadIdx.x;
     bool vis = false;
+    unsigned packed = 0u;
     if (brick < nbricks) {
-        const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
-        const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+        // at most 128^3 bricks: 32-bit index arithmetic
+        const unsigned ub = (unsigned)brick, nb = (unsigned)a.nb;
+        const int bi = (int)(ub % nb), bj = (int)((ub / nb) % nb), bk = (int)(ub / (nb * nb));
         vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
+        packed = pack_brick(bi, bj, bk);
     }
     const uint64_t m = __ballot(vis);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -293,44 +379,80 @@ __global__ __launch_bounds__(kCullThreads) void fuse_cull_kernel(FuseArgs a, int
     __syncthreads();
     if (vis)
         list[wave_base[kCullThreads / 64] + wave_base[wave] + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] =
-            (int)brick;
+            (int)packed;
 }
 
 constexpr int kFuseWaves = 4;
 
-// One wave fuses brick `brick` (lane = (j, i) inside the brick, 8 voxels along
-// z) and lowers the brick's and its super-brick's bound of F.  Returns the
-// lane's count of updated voxels.
+// The bounds a fused brick may lower (kf_device.hpp): bound l < 8 belongs to
+// brick (bi - (l & 1), bj - (l >> 1 & 1), bk - (l >> 2)), the brick itself and
+// the 7 below it whose halos reach into it; bound 8 is the super-brick's.
+// Lane l <= 8 loads bound l at the brick's start (-inf for a brick outside
+// the volume: never lowered), so the load shares the column loads' round trip.
+__device__ __forceinline__ float load_bound(const FuseArgs& a, int bi, int bj, int bk, int lane)
+{
+    float b = -__int_as_float(0x7f800000);
+    if (lane < 8) {
+        const int ni = bi - (lane & 1), nj = bj - ((lane >> 1) & 1), nk = bk - (lane >> 2);
+        if (ni >= 0 && nj >= 0 && nk >= 0) b = order_value(a.brick_key[((int64_t)nk * a.nb + nj) * a.nb + ni]);
+    } else if (lane == 8) {
+        b = order_value(a.super_key[((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3)]);
+    }
+    return b;
+}
+
+// Lowers the bounds the brick's written values fall below: every value lowers
+// its brick's and super-brick's bound, values on the local i = 0 plane (lanes
+// with i == 0) the -x neighbour's, on j = 0 (lanes with j == 0) the -y
+// neighbour's, on k = 0 (each lane's first voxel) the -z neighbour's, and the
+// values on two or three of these planes the diagonal neighbours'.  A lane
+// issues an atomicMin only where its own smallest value on the set is below
+// the bound loaded at the start; bounds are only ever lowered, so a value not
+// below the loaded bound is not below the current one either.  Once the model
+// has settled, no value is below its bounds and no atomic is issued.
+__device__ __forceinline__ void lower_bounds(const FuseArgs& a, int bi, int bj, int bk, int lane, float bound,
+                                             float lane_min, float lane_k0)
+{
+    float b[9];
+#pragma unroll
+    for (int l = 0; l < 9; ++l) b[l] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bound), l));
+    const bool i0 = (lane & 7) == 0, j0 = (lane >> 3) == 0;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const int dx = l & 1, dy = (l >> 1) & 1, dz = l >> 2;
+        const float v = dz ? lane_k0 : lane_min;
+        if ((!dx || i0) && (!dy || j0) && v < b[l])
+            atomicMin(&a.brick_key[((int64_t)(bk - dz) * a.nb + (bj - dy)) * a.nb + (bi - dx)], order_key(v));
+    }
+    if (lane_min < b[8]) atomicMin(&a.super_key[((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3)], order_key(lane_min));
+}
+
+// One wave fuses brick (bi, bj, bk) (lane = (j, i) inside the brick, its
+// column of 8 voxels along z) and lowers the bounds of F it reaches.  Returns
+// the lane's count of updated voxels.
 //
-// The brick's z table entries and its bound are uniform: they are read through
-// the constant address space (scalar loads, issued together at the start; the
-// bound is only written by this wave, after the read).
+// The brick's z table entries are uniform: they are read through the constant
+// address space (scalar loads, issued together at the start).
 typedef __attribute__((address_space(4))) const float ConstFloat;
 typedef __attribute__((address_space(4))) const int ConstInt;
 typedef __attribute__((address_space(4))) const unsigned int ConstUint;
 
 template <int FMT>
-__device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, int64_t brick, int lane,
+__device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, unsigned packed, int lane,
                                           const float* inv_count)
 {
-    const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
-    const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+    const int bi = (int)(packed & 1023u), bj = (int)((packed >> 10) & 1023u), bk = (int)(packed >> 20);
+    const int64_t brick = ((int64_t)bk * a.nb + bj) * a.nb + bi;
     float zk[8];
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) zk[kl] = ((ConstFloat*)a.zk)[bk * 8 + kl];
-    const float bmin = ((ConstFloat*)a.brick_min)[brick];
     const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
     const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
     const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
-    float fmin = __int_as_float(0x7f800000);
-    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane, fmin, inv_count);
-    // the brick's lower bound of F takes the smallest value written (one wave owns the brick)
-    fmin = wave_min(fmin);
-    if (lane == 0 && fmin < bmin) {
-        a.brick_min[brick] = fmin;
-        const int sidx = ((bk >> 3) * a.ns + (bj >> 3)) * a.ns + (bi >> 3);
-        atomicMin(&a.super_key[sidx], order_key(fmin));
-    }
+    const float bound = load_bound(a, bi, bj, bk, lane);
+    float fmin = __int_as_float(0x7f800000), fk0 = fmin;
+    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane * 8, fmin, fk0, inv_count);
+    lower_bounds(a, bi, bj, bk, lane, bound, fmin, fk0);
     return cnt;
 }
 
@@ -381,12 +503,12 @@ __global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs 
         for (int q = 0; q < 16; ++q) ip[q] = a.inv_pose[q];
     }
     const unsigned int total = *(ConstUint*)count;
-    int brick = item < cap ? ((ConstInt*)list)[item] : 0;
+    unsigned brick = item < cap ? ((ConstUint*)list)[item] : 0u;
     if (skip) return;
     int cnt = 0;
     while (item < total) {
         const unsigned int next = item + waves;
-        const int next_brick = next < cap ? ((ConstInt*)list)[next] : 0;
+        const unsigned next_brick = next < cap ? ((ConstUint*)list)[next] : 0u;
         cnt += fuse_brick<FMT>(a, ip, brick, lane, inv_count);
         item = next;
         brick = next_brick;
@@ -396,8 +518,8 @@ __global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs 
 
 // ---- many sessions per launch (batch ticks).  Jobs are sorted by brick0;
 // global brick g of the tick belongs to the job whose range holds it.  The
-// cull writes list entries job << 32 | brick (the brick's index in the job's
-// volume), so the sweep needs no search.
+// cull writes list entries job << 32 | pack_brick(bi, bj, bk) (the brick's
+// coordinates in the job's volume), so the sweep needs no search.
 constexpr int kBatchCullThreads = 256;
 
 __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(const BatchHeader* hd,
@@ -419,10 +541,9 @@ __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(cons
             const FuseArgs& a = J.a;
             if (!a.dyn->frame_skip_fuse) {
                 const float* ip = a.dyn->inv_pose;
-                const int64_t brick = g - J.brick0;
-                entry = (unsigned long long)j << 32 | (unsigned long long)brick;
-                const int bi = (int)(brick % a.nb), bj = (int)((brick / a.nb) % a.nb);
-                const int bk = (int)(brick / ((int64_t)a.nb * a.nb));
+                const unsigned brick = g - J.brick0, nb = (unsigned)a.nb;
+                const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / (nb * nb));
+                entry = (unsigned long long)j << 32 | (unsigned long long)pack_brick(bi, bj, bk);
                 vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
             }
         }
@@ -466,7 +587,7 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
     a.zk = j.a.zk;
     a.f = j.a.f;
     a.wt = j.a.wt;
-    a.brick_min = j.a.brick_min;
+    a.brick_key = j.a.brick_key;
     a.super_key = j.a.super_key;
     a.ns = j.a.ns;
     a.cull = j.a.cull;
@@ -509,7 +630,7 @@ __global__ __launch_bounds__(64 * kFuseWaves, 8) void fuse_batch_list_kernel(con
     while (item < end) {
         const unsigned long long ne = item + 1 < end ? list[item + 1] : 0ull;
         const int job = (int)(e >> 32);
-        const int64_t brick = (int64_t)(e & 0xffffffffull);
+        const unsigned brick = (unsigned)(e & 0xffffffffull);
         if (job != cur) {
             cur = job;
             load_fuse_args(cj[job], a);
@@ -527,12 +648,12 @@ __global__ __launch_bounds__(64 * kFuseWaves, 8) void fuse_batch_list_kernel(con
 
 template <int FMT>
 __global__ __launch_bounds__(256) void init_kernel(void* f, void* wt, int64_t count, float f0, float w0,
-                                                   float* brick_min, int64_t nbricks, int* super_key, int nsuper)
+                                                   int* brick_key, int64_t nbricks, int* super_key, int nsuper)
 {
     int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * 256;
     const float fb = (FMT == KF_VOXEL_F32) ? f0 : snorm_decode(snorm_encode(f0));
-    for (int64_t b = q; b < nbricks; b += stride) brick_min[b] = fb;
+    for (int64_t b = q; b < nbricks; b += stride) brick_key[b] = order_key(fb);
     for (int64_t b = q; b < nsuper; b += stride) super_key[b] = order_key(fb);
     for (; q < count; q += stride) {
         if (FMT == KF_VOXEL_F32) {
@@ -584,31 +705,35 @@ __global__ __launch_bounds__(256) void volume_to_slab(float* __restrict__ slab, 
     }
 }
 
-// Exact per-brick minimum of F (after an upload): one wave per brick.
 __global__ void super_reset_kernel(int* super_key, int nsuper)
 {
     int q = blockIdx.x * 256 + threadIdx.x;
     if (q < nsuper) super_key[q] = 0x7f800000;  // order_key(+inf)
 }
 
+// Exact halo minimum of F per brick (after an upload): one wave per brick
+// takes the 9^3 voxels [8b, 8b + 8] per axis that lie in the volume; the
+// super-brick's bound takes the smallest of its bricks'.
 template <int FMT>
-__global__ __launch_bounds__(256) void brick_min_kernel(const void* f, float* brick_min, int64_t nbricks, int nb,
-                                                        int ns, int* super_key)
+__global__ __launch_bounds__(256) void brick_bound_kernel(const void* f, int* brick_key, int64_t nbricks, int n,
+                                                          int nb, int ns, int* super_key)
 {
     const int64_t brick = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (brick >= nbricks) return;
     const int lane = threadIdx.x & 63;
+    const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / ((int64_t)nb * nb));
     float m = __int_as_float(0x7f800000);
-    for (int q = lane; q < kBrickVoxels; q += 64) {
-        const int64_t idx = brick * kBrickVoxels + q;
+    for (int q = lane; q < 9 * 9 * 9; q += 64) {
+        const int i = bi * 8 + q % 9, j = bj * 8 + (q / 9) % 9, k = bk * 8 + q / 81;
+        if (i >= n || j >= n || k >= n) continue;
+        const int64_t idx = brick_index(nb, i, j, k);
         const float v = (FMT == KF_VOXEL_F32) ? static_cast<const float*>(f)[idx]
                                               : snorm_decode(static_cast<const int16_t*>(f)[idx]);
         m = fminf(m, v);
     }
     m = wave_min(m);
     if (lane == 0) {
-        brick_min[brick] = m;
-        const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / ((int64_t)nb * nb));
+        brick_key[brick] = order_key(m);
         atomicMin(&super_key[((bk >> 3) * ns + (bj >> 3)) * ns + (bi >> 3)], order_key(m));
     }
 }
@@ -668,11 +793,12 @@ int for_each_slab(const kf_volume* v, bool to_volume, Fn&& host_side)
         const int nsuper = v->ns * v->ns * v->ns;
         super_reset_kernel<<<(nsuper + 255) / 256, 256>>>(v->super_key, nsuper);
         if (v->format == KF_VOXEL_F32)
-            brick_min_kernel<KF_VOXEL_F32><<<b, 256>>>(v->f, v->brick_min, nbricks, v->nb, v->ns, v->super_key);
+            brick_bound_kernel<KF_VOXEL_F32><<<b, 256>>>(v->f, v->brick_key, nbricks, v->n, v->nb, v->ns, v->super_key);
         else
-            brick_min_kernel<KF_VOXEL_SNORM16_U8><<<b, 256>>>(v->f, v->brick_min, nbricks, v->nb, v->ns, v->super_key);
+            brick_bound_kernel<KF_VOXEL_SNORM16_U8><<<b, 256>>>(v->f, v->brick_key, nbricks, v->n, v->nb, v->ns,
+                                                                v->super_key);
         hipError_t e = hipDeviceSynchronize();
-        if (e != hipSuccess) st = fail_hip(e, "brick_min_kernel");
+        if (e != hipSuccess) st = fail_hip(e, "brick_bound_kernel");
     }
     return st;
 }
@@ -699,7 +825,7 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
     a.f = v->f;
     a.wt = v->w;
     a.touched = v->touched;
-    a.brick_min = v->brick_min;
+    a.brick_key = v->brick_key;
     a.super_key = v->super_key;
     a.ns = v->ns;
     a.cull = is_pinhole(K) ? 1 : 0;
@@ -798,10 +924,10 @@ int launch_initialize(kf_volume* v, float f, float wt, hipStream_t s)
     const int64_t nbricks = (int64_t)v->nb * v->nb * v->nb;
     const int nsuper = v->ns * v->ns * v->ns;
     if (v->format == KF_VOXEL_F32)
-        init_kernel<KF_VOXEL_F32><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_min, nbricks, v->super_key,
+        init_kernel<KF_VOXEL_F32><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_key, nbricks, v->super_key,
                                                          nsuper);
     else
-        init_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_min, nbricks,
+        init_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 256, 0, s>>>(v->f, v->w, count, f, wt, v->brick_key, nbricks,
                                                                 v->super_key, nsuper);
     KF_LAUNCHED("init_kernel");
     return KF_OK;
@@ -837,7 +963,7 @@ int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, 
         (e = hipMalloc(&v->touched, sizeof(unsigned long long) * kFuseBlocks)) != hipSuccess ||
         (e = hipMemsetAsync(v->touched, 0, sizeof(unsigned long long) * kFuseBlocks, s)) != hipSuccess ||
         (e = hipMalloc(&v->brick_list, sizeof(int) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
-        (e = hipMalloc(&v->brick_min, sizeof(float) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
+        (e = hipMalloc(&v->brick_key, sizeof(int) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
         (e = hipMalloc(&v->super_key, sizeof(int) * (size_t)v->ns * v->ns * v->ns)) != hipSuccess ||
         (e = hipMalloc(&v->list_count, 2 * sizeof(unsigned int))) != hipSuccess ||
         (e = hipMemsetAsync(v->list_count, 0, 2 * sizeof(unsigned int), s)) != hipSuccess) {
@@ -894,7 +1020,7 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->zk);
     (void)hipFree(v->touched);
     (void)hipFree(v->brick_list);
-    (void)hipFree(v->brick_min);
+    (void)hipFree(v->brick_key);
     (void)hipFree(v->super_key);
     (void)hipFree(v->list_count);
     (void)hipFree(v->dmax);

@@ -62,7 +62,8 @@ def test_shading_and_normal_image_closed_form(kf):
                                         None, None), "normals")
     img = out.download()
     nr = nm[..., :3] @ pose[:3, :3]  # R^T n for each pixel (row vectors)
-    nr /= np.linalg.norm(nr, axis=-1, keepdims=True)
+    norm = np.linalg.norm(nr, axis=-1, keepdims=True)
+    nr = np.divide(nr, norm, out=np.zeros_like(nr), where=norm > 0)  # pixel (7, 5) holds a zero normal
     r, g, b, al = unpack(img)
     for ch, comp in ((r, 0), (g, 1), (b, 2)):
         ok = np.ones((h, w), bool)
