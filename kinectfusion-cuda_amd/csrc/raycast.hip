@@ -335,37 +335,6 @@ __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir
     return 0;
 }
 
-// Small steps are taken kGroup at a time: the group's positions come from the
-// same repeated additions, its skip tests use only the bounds a lane has
-// already cached, and the corner loads of all its evaluated samples are issued
-// together, so a group costs one memory round trip instead of kGroup.  The
-// samples are then consumed in order exactly as single steps would be (a
-// sample past an event is discarded).
-constexpr int kGroup = 2;
-
-// fast_skip without memory access: 1 when the cached super-brick or brick bound
-// is >= thr for the exact stencil, else 0 (evaluate the sample).
-__device__ __forceinline__ int cached_skip(const RaycastVolume& a, float xa, float ya, float za, float thr, int cs,
-                                           float cb, int ck, float cbk)
-{
-    const float hi = (float)a.n - 1.0f - kMargin;
-    if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return 0;
-    const int xi = (int)xa, yi = (int)ya, zi = (int)za;
-    const int sx = xi >> 6, sy = yi >> 6, sz = zi >> 6;
-    const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
-    const float shi = (float)(kSuper - 1) - kMargin;
-    if (ux >= kMargin && uy >= kMargin && uz >= kMargin && ux < shi && uy < shi && uz < shi &&
-        (sz * a.ns + sy) * a.ns + sx == cs && cb >= thr)
-        return 1;
-    const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
-    const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
-    const float bhi = (float)kBrick - kMargin;
-    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi &&
-        (bz * a.nb + by) * a.nb + bx == ck && cbk >= thr)
-        return 1;
-    return 0;
-}
-
 __device__ __forceinline__ float lane_bcast(float v, int lane)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -502,7 +471,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     const float large = mue * 1.99f, small = mue * 0.1f;
     const float invw = 1.0f / width, nf = (float)a.n;
     float step = large, prev = 0.0f, tstar = 0.0f, prev_t = 0.0f, t = nearv;
-    bool prev_skipped = false, done = false, can_defer = true;
+    bool prev_skipped = false, can_defer = true;
     int work = 0;  // trips of the march loop so far (each a skip test or an evaluation)
     int cs = -1, ck = -1;
     float cb = 0.0f, cbk = 0.0f;
@@ -529,168 +498,44 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
             }
             can_defer = false;
         }
-        if (step == large) {
-            float f = 0.0f;
-            if (march_sample<FMT>(a, o, dir, t, width, invw, nf, kSkipBound, cs, cb, ck, cbk, cc, f)) {
-                // this sample and the following ones inside the same free super-brick are
-                // skipped: positions by the same additions, no coordinates computed
-                int run = super_run(a, o, dir, t, step, invw, nf, kSkipBound, cs, cb);
-                prev_skipped = true;
+        // one sample per trip on either step: large and small steps share the code,
+        // so lanes of a wave in different phases of their march do not run two paths
+        const bool lg = step == large;
+        float f = 0.0f;
+        if (march_sample<FMT>(a, o, dir, t, width, invw, nf, lg ? kSkipBound : kSmallSkipBound, cs, cb, ck, cbk, cc,
+                              f)) {
+            // a skipped large step, and the following ones inside the same free
+            // super-brick: positions by the same additions, no coordinates computed
+            int run = lg ? super_run(a, o, dir, t, step, invw, nf, kSkipBound, cs, cb) : 0;
+            prev_skipped = true;
+            prev_t = t;
+            t = t + step;
+            for (; run > 0 && t < farv && guard < kMaxMarchSteps; --run) {
+                ++guard;
                 prev_t = t;
                 t = t + step;
-                for (; run > 0 && t < farv && guard < kMaxMarchSteps; --run) {
-                    ++guard;
-                    prev_t = t;
-                    t = t + step;
-                }
-                continue;
             }
-            if (f < 0.0f) {
-                if (t == nearv) break;
+            continue;
+        }
+        if (f < 0.0f) {
+            if (t == nearv) break;
+            if (lg) {
                 t = t - mue;  // back off and march on with small steps
                 step = small;
                 t = t + step;
                 continue;
             }
-            if (f < 0.99f) step = small;
-            prev = f;
-            prev_skipped = false;
-            t = t + step;
-            continue;
+            if (prev_skipped) {
+                const v3 pp = add3(o, mul3s(dir, prev_t));
+                prev = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
+            }
+            tstar = t - small * prev / (f - prev);
+            break;
         }
-        // a group of small samples: sample 0 at t is counted; sample k exists while
-        // the loop test and the guard would let the single-step march reach it
-        float tq[kGroup];
-        bool ex[kGroup], ev[kGroup];
-        Stencil st[kGroup];
-        tq[0] = t;
-        ex[0] = true;
-#pragma unroll
-        for (int k = 1; k < kGroup; ++k) {
-            tq[k] = tq[k - 1] + small;
-            ex[k] = ex[k - 1] && tq[k] < farv && guard + k <= kMaxMarchSteps;
-        }
-#pragma unroll
-        for (int k = 0; k < kGroup; ++k) {
-            const v3 p = add3(o, mul3s(dir, tq[k]));
-            const int sk = a.brick_key ? cached_skip(a, approx_texel(p.x, invw, nf),
-                                                     approx_texel(p.y, invw, nf),
-                                                     approx_texel(p.z, invw, nf), kSmallSkipBound, cs, cb, ck, cbk)
-                                       : 0;
-            st[k] = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
-            bool skip = sk == 1;
-            if (!skip && a.brick_key) skip = stencil_brick(a, st[k]) == ck && cbk >= kSmallSkipBound;
-            ev[k] = ex[k] && !skip;
-        }
-        // the bounds of the last existing sample's brick and super-brick (for the
-        // next group's tests; selects, no dynamic index), then the corner loads of
-        // every evaluated sample
-        if (a.brick_key) {
-            int bidx = 0, sidx = 0;
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                const Stencil& q = st[k];
-                const int bk = ((q.k0 >> 3) * a.nb + (q.j0 >> 3)) * a.nb + (q.i0 >> 3);
-                const int sk = ((q.k0 >> 6) * a.ns + (q.j0 >> 6)) * a.ns + (q.i0 >> 6);
-                bidx = ex[k] ? bk : bidx;
-                sidx = ex[k] ? sk : sidx;
-            }
-            if (bidx != ck) {
-                ck = bidx;
-                cbk = order_value(a.brick_key[bidx]);
-            }
-            if (sidx != cs) {
-                cs = sidx;
-                cb = order_value(a.super_key[sidx]);
-            }
-        }
-        // a sample whose cell is the cached cell (or an earlier sample's of this
-        // group) takes those corners; the others' loads are issued first, the
-        // copies made after them, so the group still costs one round trip
-        float cv[kGroup][8];
-        int src[kGroup];  // -2: the cached cell, -1: own loads, j: sample j of the group
-        {
-            int ki = cc.ki, kj = cc.kj, kk = cc.kk, from = -2;
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                src[k] = -1;
-                if (!ev[k]) continue;
-                const int qi = st[k].i0 | (st[k].i1 << 16), qj = st[k].j0 | (st[k].j1 << 16),
-                          qk = st[k].k0 | (st[k].k1 << 16);
-                if (qi == ki && qj == kj && qk == kk) {
-                    src[k] = from;
-                    continue;
-                }
-                uint32_t o8[8];
-                cell_offsets(a.nb, st[k], o8);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) cv[k][q] = voxel_at<FMT>(a.f, o8[q]);
-                ki = qi;
-                kj = qj;
-                kk = qk;
-                from = k;
-            }
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    float v = src[k] == -2 ? cc.c[q] : cv[k][q];
-#pragma unroll
-                    for (int jj = 0; jj < k; ++jj) v = src[k] == jj ? cv[jj][q] : v;
-                    cv[k][q] = v;
-                }
-            }
-            // the cache keeps the last evaluated sample's cell
-#pragma unroll
-            for (int k = 0; k < kGroup; ++k) {
-                if (!ev[k]) continue;
-                cc.ki = st[k].i0 | (st[k].i1 << 16);
-                cc.kj = st[k].j0 | (st[k].j1 << 16);
-                cc.kk = st[k].k0 | (st[k].k1 << 16);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) cc.c[q] = cv[k][q];
-            }
-        }
-        // consume the samples in order
-        bool stop = false;
-        t = tq[kGroup - 1] + small;  // where the march goes on if no event intervenes
-#pragma unroll
-        for (int k = 0; k < kGroup; ++k) {
-            if (stop) continue;
-            if (!ex[k]) {
-                t = tq[k];  // the loop test or the guard ends the march here
-                if (tq[k] < farv) guard = kMaxMarchSteps + 1;
-                stop = true;
-                continue;
-            }
-            if (k > 0) ++guard;
-            if (!ev[k]) {
-                prev_skipped = true;
-                prev_t = tq[k];
-                continue;
-            }
-            const float ca = st[k].a, cbw = st[k].b, cc2 = st[k].c;
-            const float e00 = (1.0f - ca) * cv[k][0] + ca * cv[k][1], e10 = (1.0f - ca) * cv[k][2] + ca * cv[k][3];
-            const float e01 = (1.0f - ca) * cv[k][4] + ca * cv[k][5], e11 = (1.0f - ca) * cv[k][6] + ca * cv[k][7];
-            const float g0 = (1.0f - cbw) * e00 + cbw * e10, g1 = (1.0f - cbw) * e01 + cbw * e11;
-            const float f = (1.0f - cc2) * g0 + cc2 * g1;
-            if (f < 0.0f) {
-                if (tq[k] != nearv) {
-                    if (prev_skipped) {
-                        const v3 pp = add3(o, mul3s(dir, prev_t));
-                        prev = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
-                    }
-                    tstar = tq[k] - small * prev / (f - prev);
-                }
-                t = tq[k];
-                stop = true;
-                done = true;
-                continue;
-            }
-            prev = f;
-            prev_skipped = false;
-        }
-        if (done || (stop && !(t < farv)) || guard > kMaxMarchSteps) break;
+        if (lg && f < 0.99f) step = small;
+        prev = f;
+        prev_skipped = false;
+        t = t + step;
     }
     write_surface<FMT>(a, L, i, j, g, tstar);
 }
