@@ -481,9 +481,18 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     for (;;) {
         if (!(t < farv)) break;
         if (++guard > kMaxMarchSteps) break;
-        if (can_defer && ++work > kWorkFloor && guard > rq.budget) {
-            const unsigned slot = atomicAdd(rq.count, 1u);
-            if (slot < rq.cap) {
+        // the lanes handing their rays over in this trip take their slots with one
+        // atomic per wave (one counter for the whole launch: per-lane atomics on it
+        // serialise)
+        const bool defer = can_defer && ++work > kWorkFloor && guard > rq.budget;
+        const uint64_t dm = __ballot(defer);
+        if (dm) {
+            const int leader = __ffsll((unsigned long long)dm) - 1;
+            unsigned base = 0u;
+            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(rq.count, (unsigned)__popcll(dm));
+            base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
+            const unsigned slot = base + (unsigned)__popcll(dm & ((1ull << (threadIdx.x & 63)) - 1ull));
+            if (defer && slot < rq.cap) {
                 DeferredRay d;
                 d.job = job;
                 d.px = j * L.w + i;
@@ -496,7 +505,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
                 rq.rays[slot] = d;
                 return;
             }
-            can_defer = false;
+            if (defer) can_defer = false;
         }
         // one sample per trip on either step: large and small steps share the code,
         // so lanes of a wave in different phases of their march do not run two paths
