@@ -77,6 +77,7 @@ struct kf_volume {
     void* f = nullptr;        // float* or int16_t* plane
     void* w = nullptr;        // float* or uint8_t* plane
     float* zk = nullptr;      // z coordinate of each k slice (repeated addition, tsdf.cu:27-31)
+    float4* zt = nullptr;     // per fuse: the inverse pose's z terms of each k slice (FuseArgs::zt)
     unsigned long long* touched = nullptr;  // per-workgroup counts of fused voxels (summed on read)
     int* brick_key = nullptr;               // per-brick halo lower bound of F, order_key form (raycast skipping)
     int* super_key = nullptr;               // per-super-brick lower bound of F (order_key)

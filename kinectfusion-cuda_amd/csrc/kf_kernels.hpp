@@ -66,6 +66,8 @@ struct FuseArgs {
     float half, res, mue;
     float inv_mue;  // 1 / mue correctly rounded (the exact reciprocal the quotient diff / mue is formed from)
     const float* zk;
+    float4* zt;     // (m2 z + m3) per row of the inverse pose for each k slice, written by the fuse's cull
+    float band;     // near-integer band of the projection (2^-21 max(w, h))
     void* f;
     void* wt;
     unsigned long long* touched;

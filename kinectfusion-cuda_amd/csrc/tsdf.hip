@@ -139,11 +139,13 @@ __device__ __forceinline__ float div_by_exact_reciprocal(float x, float b, float
 
 // True when x may lie within a few ulp of an integer (0 and the image size
 // included): x*rcp(z) is within 2^-22 |x/z| of the rounded quotient x/z
-// (v_rcp_f32 is within 1 ulp), so outside this band both have the same sign,
-// the same integer part and the same side of every integer bound.
-__device__ __forceinline__ bool near_integer(float x)
+// (v_rcp_f32 is within 1 ulp), so outside a band of half-width band =
+// 2^-21 max(w, h) both have the same sign, the same integer part and the same
+// side of every integer bound wherever |x| <= 2 max(w, h); beyond that both
+// lie outside the image, and the integer part is not used.
+__device__ __forceinline__ bool near_integer(float x, float band)
 {
-    return fabsf(x - rintf(x)) <= fmaf(fabsf(x), 0x1p-20f, 0x1p-40f);
+    return fabsf(x - rintf(x)) <= band;
 }
 
 // A brick column's planes in registers: the 8 voxels (i, j, 0..7) are
@@ -235,21 +237,20 @@ struct ColumnPlanes<KF_VOXEL_SNORM16_U8> {
 // any was updated (the others keep the bits they were read with; one wave owns
 // the brick).
 template <int FMT>
-__device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v3 col, const float* zk, int64_t base,
-                                           float& fmin, float& fk0, const float* inv_count)
+__device__ __forceinline__ int fuse_column(const FuseArgs& a, v3 col, const float4* zt, int64_t base, float& fmin,
+                                           float& fk0, const float* inv_count)
 {
     float dep[8], pz[8];
-    int pix[8];
+    unsigned pix[8];
     bool hit[8];
     const float fw = (float)a.w, fh = (float)a.h;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
-        const float z = zk[kl];
-        // pe = (m0*x + m1*y) + (m2*z + m3*1): the first pair is shared by the column
-        v3 pe = mk3(col.x + (ip[8] * z + ip[12] * 1.0f), col.y + (ip[9] * z + ip[13] * 1.0f),
-                    col.z + (ip[10] * z + ip[14] * 1.0f));
+        // pe = (m0*x + m1*y) + (m2*z + m3*1): the first pair is shared by the
+        // column, the second by the slice (zt, formed once per fuse)
+        v3 pe = mk3(col.x + zt[kl].x, col.y + zt[kl].y, col.z + zt[kl].z);
         bool ok = pe.z >= kMinDistance && pe.z < kMaxDistance;
-        int q = 0;
+        unsigned q = 0u;
         if (ok) {
             float px, py;
             if (a.cull) {
@@ -259,7 +260,7 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
                 const float r = __builtin_amdgcn_rcpf(pe.z);
                 px = vx * r;
                 py = vy * r;
-                if (near_integer(px) || near_integer(py)) {
+                if (near_integer(px, a.band) || near_integer(py, a.band)) {
                     px = vx / pe.z;
                     py = vy / pe.z;
                 }
@@ -269,7 +270,8 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, const float* ip, v
                 py = pv.y / pv.z;
             }
             ok = px >= 0.0f && px < fw && py >= 0.0f && py < fh;
-            if (ok) q = (int)py * a.w + (int)px;
+            // 0 <= px < w, 0 <= py < h: a 24-bit product (images are below 2^24 pixels a side)
+            if (ok) q = __umul24((unsigned)py, (unsigned)a.w) + (unsigned)px;
         }
         hit[kl] = ok;
         pix[kl] = q;
@@ -339,6 +341,15 @@ __device__ __forceinline__ void load_inv_pose(const FuseArgs& a, float* ip)
     __syncthreads();
 }
 
+// The z terms of slice k's projection, (m2 z + m3 * 1) per row of the inverse
+// pose (the second pair of the reference's (m0 x + m1 y) + (m2 z + m3 * 1)),
+// formed by the cull for every slice once per fuse and read by the sweep.
+__device__ __forceinline__ void z_terms(const float* ip, const float* zk, float4* zt, int k)
+{
+    const float z = zk[k];
+    zt[k] = make_float4(ip[8] * z + ip[12] * 1.0f, ip[9] * z + ip[13] * 1.0f, ip[10] * z + ip[14] * 1.0f, 0.0f);
+}
+
 // Pass 1: one lane per brick; bricks whose bounding sphere meets the view
 // frustum are appended to `list`, one atomic per workgroup.  The list length
 // lives in count[parity]; this pass also clears count[1 - parity], which the
@@ -353,6 +364,7 @@ __global__ __launch_bounds__(kCullThreads) void fuse_cull_kernel(FuseArgs a, int
     __shared__ unsigned int wave_base[kCullThreads / 64 + 1];
     load_inv_pose(a, ip);
     if (blockIdx.x == 0 && threadIdx.x == 0) count[1 - parity] = 0u;
+    for (int k = blockIdx.x * kCullThreads + threadIdx.x; k < a.n; k += gridDim.x * kCullThreads) z_terms(ip, a.zk, a.zt, k);
     const int64_t brick = (int64_t)blockIdx.x * kCullThreads + threadIdx.x;
     bool vis = false;
     unsigned packed = 0u;
@@ -431,8 +443,8 @@ __device__ __forceinline__ void lower_bounds(const FuseArgs& a, int bi, int bj, 
 // column of 8 voxels along z) and lowers the bounds of F it reaches.  Returns
 // the lane's count of updated voxels.
 //
-// The brick's z table entries are uniform: they are read through the constant
-// address space (scalar loads, issued together at the start).
+// The brick's z terms are uniform: they are read through the constant address
+// space (scalar loads, issued together at the start).
 typedef __attribute__((address_space(4))) const float ConstFloat;
 typedef __attribute__((address_space(4))) const int ConstInt;
 typedef __attribute__((address_space(4))) const unsigned int ConstUint;
@@ -443,15 +455,18 @@ __device__ __forceinline__ int fuse_brick(const FuseArgs& a, const float* ip, un
 {
     const int bi = (int)(packed & 1023u), bj = (int)((packed >> 10) & 1023u), bk = (int)(packed >> 20);
     const int64_t brick = ((int64_t)bk * a.nb + bj) * a.nb + bi;
-    float zk[8];
+    float4 zt[8];
 #pragma unroll
-    for (int kl = 0; kl < 8; ++kl) zk[kl] = ((ConstFloat*)a.zk)[bk * 8 + kl];
+    for (int kl = 0; kl < 8; ++kl) {
+        ConstFloat* z = (ConstFloat*)a.zt + (bk * 8 + kl) * 4;
+        zt[kl] = make_float4(z[0], z[1], z[2], 0.0f);
+    }
     const int i = bi * 8 + (lane & 7), j = bj * 8 + (lane >> 3);
     const float x = a.half + (float)i * a.res, y = a.half + (float)j * a.res;
     const v3 col = mk3(ip[0] * x + ip[4] * y, ip[1] * x + ip[5] * y, ip[2] * x + ip[6] * y);
     const float bound = load_bound(a, bi, bj, bk, lane);
     float fmin = __int_as_float(0x7f800000), fk0 = fmin;
-    const int cnt = fuse_column<FMT>(a, ip, col, zk, brick * kBrickVoxels + lane * 8, fmin, fk0, inv_count);
+    const int cnt = fuse_column<FMT>(a, col, zt, brick * kBrickVoxels + lane * 8, fmin, fk0, inv_count);
     lower_bounds(a, bi, bj, bk, lane, bound, fmin, fk0);
     return cnt;
 }
@@ -542,6 +557,7 @@ __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(cons
             if (!a.dyn->frame_skip_fuse) {
                 const float* ip = a.dyn->inv_pose;
                 const unsigned brick = g - J.brick0, nb = (unsigned)a.nb;
+                for (unsigned k = brick; k < (unsigned)a.n; k += nb * nb * nb) z_terms(ip, a.zk, a.zt, (int)k);
                 const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / (nb * nb));
                 entry = (unsigned long long)j << 32 | (unsigned long long)pack_brick(bi, bj, bk);
                 vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
@@ -585,6 +601,8 @@ __device__ __forceinline__ void load_fuse_args(const CJ& j, FuseArgs& a)
     a.mue = j.a.mue;
     a.inv_mue = j.a.inv_mue;
     a.zk = j.a.zk;
+    a.zt = j.a.zt;
+    a.band = j.a.band;
     a.f = j.a.f;
     a.wt = j.a.wt;
     a.brick_key = j.a.brick_key;
@@ -822,6 +840,8 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
     a.mue = v->mue;
     a.inv_mue = 1.0f / v->mue;
     a.zk = v->zk;
+    a.zt = v->zt;
+    a.band = (float)std::max(w, h) * 0x1p-21f;
     a.f = v->f;
     a.wt = v->w;
     a.touched = v->touched;
@@ -960,6 +980,7 @@ int volume_create(int n, float width_m, float mue_m, int format, hipStream_t s, 
     hipError_t e;
     if ((e = hipMalloc(&v->f, (size_t)count * fb)) != hipSuccess || (e = hipMalloc(&v->w, (size_t)count * wb)) != hipSuccess ||
         (e = hipMalloc(&v->zk, sizeof(float) * n)) != hipSuccess ||
+        (e = hipMalloc(&v->zt, sizeof(float4) * n)) != hipSuccess ||
         (e = hipMalloc(&v->touched, sizeof(unsigned long long) * kFuseBlocks)) != hipSuccess ||
         (e = hipMemsetAsync(v->touched, 0, sizeof(unsigned long long) * kFuseBlocks, s)) != hipSuccess ||
         (e = hipMalloc(&v->brick_list, sizeof(int) * (size_t)v->nb * v->nb * v->nb)) != hipSuccess ||
@@ -1018,6 +1039,7 @@ int kf_volume_destroy(kf_volume* v)
     (void)hipFree(v->f);
     (void)hipFree(v->w);
     (void)hipFree(v->zk);
+    (void)hipFree(v->zt);
     (void)hipFree(v->touched);
     (void)hipFree(v->brick_list);
     (void)hipFree(v->brick_key);
