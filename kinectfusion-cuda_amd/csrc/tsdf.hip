@@ -117,13 +117,6 @@ __global__ __launch_bounds__(256) void depth_max_kernel(const float* __restrict_
     }
 }
 
-// tsdf.cu:20-75 for the 8 voxels of one brick column (lane = (j, i) of the
-// brick, kl = 0..7 along z).  The projection of all 8 voxels is computed first;
-// then, for every voxel that projects into the image, the depth gather and the
-// voxel's own f and w loads are issued together (the voxel loads do not wait
-// for the depth test, so a brick costs one memory round trip, not two).  A
-// voxel whose depth test fails is read but not written.  Returns the number of
-// updated voxels.
 // x / b rounded to nearest from y = RN(1/b): q = RN(x*y), r = x - b*q (exact by
 // the fused multiply-add), RN(q + r*y) is RN(x/b) for every x and b whose
 // quotient and residual neither overflow nor underflow (Markstein's theorem;
@@ -223,7 +216,8 @@ struct ColumnPlanes<KF_VOXEL_SNORM16_U8> {
     }
 };
 
-// The 8 voxels (one per z-slice kl) of one lane's column of a brick.
+// tsdf.cu:20-75 for the 8 voxels (one per z-slice kl) of one lane's column of
+// a brick (lane = (j, i) of the brick); returns the number of updated voxels.
 // Projection (tsdf.cu:34-45): pe = inv_pose * (x, y, z, 1), pixel = K pe / pe.z;
 // only the pixel's integer part and the range test use the quotient, so it is
 // formed from the hardware reciprocal and recomputed by the correctly rounded
@@ -233,9 +227,9 @@ struct ColumnPlanes<KF_VOXEL_SNORM16_U8> {
 // reciprocal of 1..129 from inv_count (LDS); every value written is the one the
 // plain quotients give.  fmin takes the smallest value written, fk0 the value
 // written to voxel 0 (the brick's k = 0 plane).  The column's planes are read
-// when any of its voxels projects into the image and written back whole when
-// any was updated (the others keep the bits they were read with; one wave owns
-// the brick).
+// when any of its voxels passes the depth test (after the depth gathers) and
+// written back whole when any was updated (the others keep the bits they were
+// read with; one wave owns the brick).
 template <int FMT>
 __device__ __forceinline__ int fuse_column(const FuseArgs& a, v3 col, const float4* zt, int64_t base, float& fmin,
                                            float& fk0, const float* inv_count)
@@ -277,12 +271,18 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, v3 col, const floa
         pix[kl] = q;
         pz[kl] = pe.z;
     }
-    bool any = false;
 #pragma unroll
     for (int kl = 0; kl < 8; ++kl) {
         dep[kl] = 0.0f;
         if (hit[kl]) dep[kl] = a.depth[pix[kl]];
-        any = any || hit[kl];
+    }
+    // the depth test first: the planes of a column none of whose voxels is
+    // updated (behind the surface or on invalid depth) are not read
+    bool upd[8], any = false;
+#pragma unroll
+    for (int kl = 0; kl < 8; ++kl) {
+        upd[kl] = hit[kl] && is_depth_valid(dep[kl]) && dep[kl] - pz[kl] >= -a.mue;
+        any = any || upd[kl];
     }
     ColumnPlanes<FMT> v;
     v.clear();
@@ -292,7 +292,7 @@ __device__ __forceinline__ int fuse_column(const FuseArgs& a, v3 col, const floa
     for (int kl = 0; kl < 8; ++kl) {
         const float d = dep[kl];
         const float diff = d - pz[kl];
-        if (!(hit[kl] && is_depth_valid(d) && diff >= -a.mue)) continue;
+        if (!upd[kl]) continue;
         // f = min(1, diff / mue): diff >= mue makes the rounded quotient >= 1, so the
         // division (most updated voxels lie in front of the surface) is skipped there
         float f = 1.0f;
