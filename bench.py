@@ -21,6 +21,10 @@ timed region.  `roofline_raycast` is the raycast (first pass + finishing pass,
 voxels the reference march's samples read (counted by the oracle on the model
 after the timed frames).  "batched" and "serving" report the multi-session
 forms (see their docstrings); every input is resident before timing.
+"precision" runs the precision form on config 3's workload and compares it
+with the FP64 oracle of the same algorithm.  The oracle runs only in the CPU leg
+at the end (cpu_leg): the CPU baseline, the |S| counts and the FP64 comparison;
+no GPU figure is measured while it runs.
 """
 import argparse
 import json
@@ -58,6 +62,8 @@ def parse():
                     help="length of the served 512^3 load offered near capacity")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-footprint", action="store_true", help="skip the oracle's raycast footprint counts")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision form's run against the FP64 oracle (config 3's workload)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (the known-size copy that "
                          "scales HBM counters in counter-collection runs); not part of the JSON line")
@@ -72,14 +78,6 @@ def rot_err_deg(pa, pb):
     M = A.T @ B
     w = 0.5 * np.array([M[2, 1] - M[1, 2], M[0, 2] - M[2, 0], M[1, 0] - M[0, 1]])
     return float(np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(M) - 1.0) / 2.0)))
-
-
-def footprint(volume_f, n, L, mue, pose):
-    """Distinct voxels and samples of the reference march at pose over the 3 levels (oracle)."""
-    from oracle.kfref import Oracle
-    o = Oracle()
-    kis = [o.intrinsics(525.0, 525.0, 319.5, 239.5, lvl)[1] for lvl in range(3)]
-    return o.raycast_footprint(volume_f, n, L, mue, pose, kis, W, H)
 
 
 class Events:
@@ -183,28 +181,13 @@ def batched(args, kfusion, stream):
         errs.append(float(np.abs(p[12:15] - seqs[s].pose(last)[12:15]).max()))
         rerr.append(rot_err_deg(p, seqs[s].pose(last)))
     failed = sum(1 for i in ids if b.status(i) != 0)
-    # raycast footprint of a sample of sessions (the models after the last tick, at their poses)
-    ray_roof = None
-    if stage and not args.no_footprint:
+    # the models after the last tick of a sample of sessions, at their poses: the CPU leg counts
+    # the reference march's footprint on them for the raycast roofline
+    ray_models = None
+    if stage and not args.no_footprint and not args.no_cpu_baseline:
         sample = list(range(0, S, max(1, S // 4)))[:4]
-        distinct = samples = 0
-        for s in sample:
-            F, _ = b.volume_download(ids[s])
-            d, sm = footprint(F, n, L, mue, b.pose(ids[s]))
-            distinct += d
-            samples += sm
-            del F
-        scale = S / len(sample)
-        alg = distinct * scale * 2.0 + S * LEVEL_PIXELS * 32.0
-        rms = (per[1] + per[2]) * 1e-3
-        ray_roof = {"bound": "hbm", "kernel": "raycast_batch_kernel + raycast_batch_finish_kernel (3 levels, "
-                                              "all sessions)",
-                    "achieved": round(alg / rms / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(alg / rms / 1e9 / HBM_PEAK_GBS, 5), "avg_launch_ms": round(float(rms * 1e3), 4),
-                    "distinct_voxels_per_tick": round(distinct * scale), "algorithmic_bytes_per_launch": round(alg),
-                    "samples_per_s": round(samples * scale / rms, 1),
-                    "sample": f"|S| and samples counted by the oracle on sessions {sample}, scaled by {scale:g}",
-                    "bytes_per_unit": "2 B per distinct voxel read (snorm16 f) + 32 B per pixel (vertex + normal)"}
+        ray_models = {"models": [(b.volume_download(ids[s])[0], b.pose(ids[s])) for s in sample], "sample": sample,
+                      "sessions": S, "ms": float(per[1] + per[2])}
     shape, tuned = b.shape()
     sb = kfusion.session_bytes(kfusion.SessionConfig(W, H, volume_n=n, volume_size=L, mue=mue))
     b.close()
@@ -224,9 +207,7 @@ def batched(args, kfusion, stream):
     if stage:
         out["stage_ms_per_tick"] = stage
         out["roofline_fuse"] = fuse_roof
-    if ray_roof:
-        out["roofline_raycast"] = ray_roof
-    return out
+    return out, ray_models
 
 
 def recorded_traffic():
@@ -396,6 +377,109 @@ def copy_peak(kfusion, stream):
     return round(2.0 * nb / (best * 1e-3) / 1e9, 1)
 
 
+PREC_FRAMES, PREC_N, PREC_L = 30, 256, 3.0
+
+
+def precision_run(kfusion):
+    """The precision form on config 3's workload: seed-1 frames, 640x480 -> 256^3 snorm16 + uint8
+    counts over a 3 m cube, Kahan-compensated ICP sums and FP64 solve, 30 frames.  Returns what the
+    CPU leg compares with the FP64 oracle of the same algorithm: every frame's pose, the final
+    volume and the predicted vertex maps."""
+    import numpy as np
+
+    from kfusion.synth import SyntheticSequence
+    seq = SyntheticSequence(seed=1, n_frames=PREC_FRAMES, L=PREC_L)
+    raws = seq.frames()
+    th = float(np.deg2rad(20.0))
+    gp = kfusion.Pipeline(W, H, seq.fx, seq.fy, seq.cx, seq.cy, 1e-3, PREC_N, PREC_L, 0.03,
+                          voxel_format=kfusion.KF_VOXEL_SNORM16_U8, initial_pose=seq.pose(0),
+                          icp=kfusion.IcpConfig((10, 5, 4), W, H, 0.1, th, compensated=True))
+    dev = kfusion.DeviceArray.from_host(raws)
+    poses, status = [], 0
+    for f in range(PREC_FRAMES):
+        gp.frame(dev.ptr.value + f * W * H * 2)
+        status = status or gp.sync()
+        poses.append(gp.pose())
+    F, Wt = gp.volume_download()
+    maps = [gp.map(4, lvl) for lvl in range(3)]
+    gp.close()
+    dev.free()
+    return {"seq": seq, "raws": raws, "poses": poses, "F": F, "W": Wt, "maps": maps, "status": status}
+
+
+def cpu_leg(result, args, seq, raws, n, L, mue, plain_model, batch_models, prec):
+    """Everything that runs the oracle, after the GPU legs: the CPU baseline (cpu_baseline), the
+    reference march's footprint |S| on the GPU's models for the raycast rooflines (the oracle as a
+    counter of algorithmic bytes), and the precision form against the FP64 oracle of the same
+    algorithm (the oracle as the checker)."""
+    import numpy as np
+    from oracle.kfref import Oracle, OraclePipeline
+
+    result["cpu_baseline"] = cpu_baseline(seq, raws, n, L, mue, args.cpu_seconds)
+    o = Oracle()
+    kis = [o.intrinsics(525.0, 525.0, 319.5, 239.5, lvl)[1] for lvl in range(3)]
+    if plain_model is not None:
+        F, pose, ray_ms = plain_model
+        distinct, samples = o.raycast_footprint(F, n, L, mue, pose, kis, W, H)
+        rms = ray_ms * 1e-3
+        rb = distinct * 4.0 + LEVEL_PIXELS * 32.0
+        result["roofline_raycast"] = {
+            "bound": "hbm", "kernel": "raycast_kernel + raycast_finish_kernel (3 levels)",
+            "achieved": round(rb / rms / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(rb / rms / 1e9 / HBM_PEAK_GBS, 5), "avg_launch_ms": round(ray_ms, 5),
+            "distinct_voxels": distinct, "algorithmic_bytes_per_launch": round(rb),
+            "samples_per_frame": samples, "samples_per_s": round(samples / rms, 1),
+            "bytes_per_unit": "4 B per distinct voxel the reference march reads (f32 f, trilinear and gradient "
+                              "stencils) + 32 B per pixel (vertex + normal)",
+            "sample": "|S| counted by the oracle on the model after the timed frames at the final pose"}
+    if batch_models is not None and "batched" in result:
+        distinct = samples = 0
+        for F, pose in batch_models["models"]:
+            d, sm = o.raycast_footprint(F, n, L, mue, pose, kis, W, H)
+            distinct += d
+            samples += sm
+        S, sample = batch_models["sessions"], batch_models["sample"]
+        scale = S / len(sample)
+        alg = distinct * scale * 2.0 + S * LEVEL_PIXELS * 32.0
+        rms = batch_models["ms"] * 1e-3
+        result["batched"]["roofline_raycast"] = {
+            "bound": "hbm", "kernel": "raycast_batch_kernel + raycast_batch_finish_kernel (3 levels, all sessions)",
+            "achieved": round(alg / rms / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / rms / 1e9 / HBM_PEAK_GBS, 5), "avg_launch_ms": round(float(rms * 1e3), 4),
+            "distinct_voxels_per_tick": round(distinct * scale), "algorithmic_bytes_per_launch": round(alg),
+            "samples_per_s": round(samples * scale / rms, 1),
+            "sample": f"|S| and samples counted by the oracle on sessions {sample}, scaled by {scale:g}",
+            "bytes_per_unit": "2 B per distinct voxel read (snorm16 f) + 32 B per pixel (vertex + normal)"}
+    if prec is not None:
+        ps = prec["seq"]
+        op = OraclePipeline(W, H, PREC_N, PREC_L, 0.03, ps.fx, ps.fy, ps.cx, ps.cy, 1e-3, (10, 5, 4), 0.1,
+                            float(np.deg2rad(20.0)), init_pose=ps.pose(0), dtype=np.float64, count_weights=True)
+        rot, trans = [], []
+        for f in range(PREC_FRAMES):
+            op.frame(prec["raws"][f])
+            a, b = np.asarray(prec["poses"][f], np.float64), np.asarray(op.pose(), np.float64)
+            rot.append(rot_err_deg(a, b) * np.pi / 180.0)
+            trans.append(float(np.abs(a[12:15] - b[12:15]).max()))
+        F64, W64 = op.tsdf()
+        both = (prec["W"] > 0) & (W64 > 0)
+        dF = np.abs(prec["F"].astype(np.float64) - F64)[both]
+        out = {"workload": f"config 3: seed-1 frames, 640x480 -> {PREC_N}^3 snorm16 + uint8 counts over "
+                           f"{PREC_L:g} m, Kahan-compensated ICP sums, FP64 solve, {PREC_FRAMES} frames",
+               "reference": "FP64 oracle of the same algorithm (count weights, unquantised F), run independently",
+               "icp_status": int(prec["status"]),
+               "pose_max_rot_rad": float(f"{max(rot):.3g}"), "pose_max_trans_m": float(f"{max(trans):.3g}"),
+               "tsdf_rms_mue": float(f"{np.sqrt(np.mean(dF ** 2)):.3g}"),
+               "tsdf_within_1e-3_mue": round(float((dF <= 1e-3).mean()), 5)}
+        for lvl in range(3):
+            vg, v64 = prec["maps"][lvl], op.map(1, lvl)
+            ok = (vg[..., 3] == -1) & (v64[..., 3] == -1)
+            dz = np.linalg.norm(vg[..., :3].astype(np.float64) - v64[..., :3], axis=-1)[ok]
+            out[f"raycast_L{lvl}_within_1mm"] = round(float((dz <= 1e-3).mean()), 5)
+            out[f"raycast_L{lvl}_mean_mm"] = round(1e3 * float(dz.mean()), 4)
+        op.close()
+        result["precision"] = out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -495,42 +579,35 @@ def main():
                      "bytes_per_unit": "16 B per touched voxel (f32 f + f32 w, read and written) + 4 B per depth "
                                        "pixel"},
     }
-    if not args.no_footprint:
-        # the model the next frame's raycast would see, at the current pose
+    plain_model = None
+    if not args.no_footprint and not args.no_cpu_baseline:
+        # the model the next frame's raycast would see, at the current pose (the CPU leg counts the
+        # reference march's footprint on it)
         Fv = np.empty(n ** 3 * 2, np.float32)
         kfusion.check(lib.kf_volume_download(vol, Fv.ctypes.data_as(C.c_void_p)), "volume download")
-        F = np.ascontiguousarray(Fv[0::2])
+        plain_model = (np.ascontiguousarray(Fv[0::2]), pose.copy(), stage_ms["raycast"])
         del Fv
-        distinct, samples = footprint(F, n, L, mue, pose)
-        del F
-        rms = stage_ms["raycast"] * 1e-3
-        rb = distinct * 4.0 + LEVEL_PIXELS * 32.0
-        result["roofline_raycast"] = {
-            "bound": "hbm", "kernel": "raycast_kernel + raycast_finish_kernel (3 levels)",
-            "achieved": round(rb / rms / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(rb / rms / 1e9 / HBM_PEAK_GBS, 5), "avg_launch_ms": round(stage_ms["raycast"], 5),
-            "distinct_voxels": distinct, "algorithmic_bytes_per_launch": round(rb),
-            "samples_per_frame": samples, "samples_per_s": round(samples / rms, 1),
-            "bytes_per_unit": "4 B per distinct voxel the reference march reads (f32 f, trilinear and gradient "
-                              "stencils) + 32 B per pixel (vertex + normal)",
-            "sample": "|S| counted by the oracle on the model after the timed frames at the final pose"}
     lib.kf_pipeline_destroy(h)
     dev.free()
     result["hbm_copy_gbs"] = copy_peak(kfusion, stream)
     result["roofline"]["frac_of_copy"] = round(achieved / result["hbm_copy_gbs"], 4)
     shape = None
+    batch_models = None
     if not args.no_batch:
-        bt = batched(args, kfusion, stream)
+        bt, batch_models = batched(args, kfusion, stream)
         bt["vs_plain"] = round(bt["value"] / result["value"], 3)
         result["batched"] = bt
         shape = bt["launch_shape"]
+    prec = None
+    if not args.no_precision and not args.no_cpu_baseline:
+        prec = precision_run(kfusion)
     if not args.no_serve:
         sv = serving(args, kfusion, shape, result.get("batched", {}).get("value"))
         sv["near_capacity_512"]["steady_vs_plain"] = round(sv["near_capacity_512"]["steady_offered_frames_per_s"] /
                                                            result["value"], 3)
         result["serving"] = sv
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(seq, raws, n, L, mue, args.cpu_seconds)
+        cpu_leg(result, args, seq, raws, n, L, mue, plain_model, batch_models, prec)
     print(json.dumps(result), flush=True)
     if args.copy_mb > 0:
         nb = args.copy_mb << 20
