@@ -146,8 +146,8 @@ int alloc(void** p, size_t bytes, size_t* acc)
 size_t volume_meta_bytes(int n)
 {
     const size_t nb = (size_t)n / 8, ns = (nb + 7) / 8;
-    return sizeof(float) * n + sizeof(unsigned long long) * kf::kFuseBlocks + nb * nb * nb * (sizeof(int) + sizeof(float)) +
-           ns * ns * ns * sizeof(int) + 2 * sizeof(unsigned);
+    return (sizeof(float) + sizeof(float4)) * n + sizeof(unsigned long long) * kf::kFuseBlocks +
+           nb * nb * nb * (sizeof(int) + sizeof(int)) + ns * ns * ns * sizeof(int) + 2 * sizeof(unsigned);
 }
 
 size_t maps_bytes(int w, int h)
