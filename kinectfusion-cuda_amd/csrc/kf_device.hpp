@@ -272,13 +272,22 @@ __device__ __forceinline__ int64_t brick_index(int nb, int i, int j, int k)
 // cell then cost 6 terms and 12 additions, and a load is a 32-bit offset from
 // the plane's base.
 __device__ __forceinline__ uint32_t brick_ox(int i) { return (uint32_t)(i >> 3) * 512u + (uint32_t)(i & 7) * 8u; }
+// The row and slab strides of the brick grid, 512 nb and 512 nb^2, stay below
+// 2^24 (nb <= 128), so the products use the 24-bit multiply (full rate; a
+// 32-bit multiply issues in four passes).
 __device__ __forceinline__ uint32_t brick_oy(int nb, int j)
 {
-    return (uint32_t)(j >> 3) * (uint32_t)nb * 512u + (uint32_t)(j & 7) * 64u;
+    return __umul24((uint32_t)(j >> 3), (uint32_t)nb << 9) + (uint32_t)(j & 7) * 64u;
 }
 __device__ __forceinline__ uint32_t brick_oz(int nb, int k)
 {
-    return (uint32_t)(k >> 3) * (uint32_t)nb * (uint32_t)nb * 512u + (uint32_t)(k & 7);
+    return __umul24((uint32_t)(k >> 3), ((uint32_t)nb * (uint32_t)nb) << 9) + (uint32_t)(k & 7);
+}
+// Linear index of cell (x, y, z) of an m^3 grid (bricks or super-bricks:
+// m <= 128), in 24-bit multiplies.
+__device__ __forceinline__ int grid_index(int x, int y, int z, int m)
+{
+    return (int)(__umul24(__umul24((uint32_t)z, (uint32_t)m) + (uint32_t)y, (uint32_t)m) + (uint32_t)x);
 }
 
 // Brick bounds (raycast skipping): brick (bi, bj, bk) keeps a lower bound of F

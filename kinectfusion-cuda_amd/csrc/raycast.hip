@@ -171,7 +171,7 @@ __device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float
 // Brick holding the stencil's first corner (its halo bound covers the stencil).
 __device__ __forceinline__ int stencil_brick(const RaycastVolume& a, const Stencil& t)
 {
-    return ((t.k0 >> 3) * a.nb + (t.j0 >> 3)) * a.nb + (t.i0 >> 3);
+    return grid_index(t.i0 >> 3, t.j0 >> 3, t.k0 >> 3, a.nb);
 }
 
 // True when the sample's bound is >= thr (the exact stencil), through the
@@ -204,7 +204,7 @@ __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float
     const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
     const float shi = (float)(kSuper - 1) - kMargin;
     if (ux >= kMargin && uy >= kMargin && uz >= kMargin && ux < shi && uy < shi && uz < shi) {
-        const int sidx = (sz * a.ns + sy) * a.ns + sx;
+        const int sidx = grid_index(sx, sy, sz, a.ns);
         if (sidx != cs) {
             cs = sidx;
             cb = order_value(a.super_key[sidx]);
@@ -217,7 +217,7 @@ __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float
     const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
     const float bhi = (float)kBrick - kMargin;
     if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi) {
-        const int bidx = (bz * a.nb + by) * a.nb + bx;
+        const int bidx = grid_index(bx, by, bz, a.nb);
         if (bidx != ck) {
             ck = bidx;
             cbk = order_value(a.brick_key[bidx]);
@@ -261,12 +261,16 @@ __device__ __forceinline__ int region_run(v3 o, v3 dir, float t, float step, flo
 
 // Samples after the one at t inside the cached super-brick, which covers
 // stencils inside it; 0 when its bound is below thr or the sample at t is
-// near its faces.
+// near its faces.  The super-brick's coordinates are those of the sample's
+// position when its index is the cached one (region_run checks the margins).
 __device__ __forceinline__ int super_run(const RaycastVolume& a, v3 o, v3 dir, float t, float step, float invw,
                                          float nf, float thr, int cs, float cb)
 {
     if (cs < 0 || !(cb >= thr)) return 0;
-    const int sc[3] = {cs % a.ns, (cs / a.ns) % a.ns, cs / (a.ns * a.ns)};
+    const v3 p = add3(o, mul3s(dir, t));
+    const int sc[3] = {(int)approx_texel(p.x, invw, nf) / kSuper, (int)approx_texel(p.y, invw, nf) / kSuper,
+                       (int)approx_texel(p.z, invw, nf) / kSuper};
+    if (sc[0] < 0 || sc[1] < 0 || sc[2] < 0 || grid_index(sc[0], sc[1], sc[2], a.ns) != cs) return 0;
     int lo[3], hi[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
