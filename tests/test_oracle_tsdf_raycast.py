@@ -177,3 +177,31 @@ def test_raycast_footprint_counts(oracle32):
     assert s_wall > 0 and d_wall <= 8 * s_wall
     codes = np.where(F2 > 1.0, -32768, np.rint(np.clip(F2, -1, 1) * 32767)).astype(np.int16)
     assert oracle32.raycast_footprint(codes, n, L, mue, pose, [Ki], w, h) == (d_wall, s_wall)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fuse_counts_closed_form(dtype):
+    """The count-weight update (the precision form's rule on unquantised planes, the FP64 error
+    reference): after walls at depths D1 then D2, an observed voxel holds the mean of the two
+    truncated values and W = 2; after 130 frames W saturates at 128.  A wrong weight, a missing
+    +1 or an uncapped count fails it."""
+    o = kfref.Oracle(dtype)
+    n, Wd, mue = 32, 2.0, 0.1
+    K, _ = o.intrinsics(60.0, 60.0, 31.5, 23.5, 0)
+    d1 = np.full((48, 64), 0.8, np.float32)
+    d2 = np.full((48, 64), 0.85, np.float32)
+    F, W = o.new_volume(n)
+    o.fuse_counts(F, W, n, Wd, mue, d1, I4, K)
+    seen1 = W > 0
+    f1 = F.copy()
+    assert (W[seen1] == 1).all() and (F[~seen1] == 2.0).all()
+    _, fa = numpy_fuse_reference(n, Wd, mue, 0.8, K)
+    np.testing.assert_allclose(f1[seen1], fa[seen1], atol=2e-5)
+    o.fuse_counts(F, W, n, Wd, mue, d2, I4, K)
+    _, fb = numpy_fuse_reference(n, Wd, mue, 0.85, K)
+    both = seen1 & (W == 2)
+    assert both.sum() > 500
+    np.testing.assert_allclose(F[both], 0.5 * (fa[both] + fb[both]), atol=4e-5)
+    for _ in range(128):
+        o.fuse_counts(F, W, n, Wd, mue, d2, I4, K)
+    assert W.max() == 128 and (W[both] == 128).all()
