@@ -193,13 +193,25 @@ __device__ __forceinline__ bool skippable(const RaycastVolume& a, const Stencil&
 // corner is in one brick whose bound is below it (so skippable() would say
 // no), -1 when the margins leave it open.  The last super-brick's and brick's
 // bounds are cached in (cs, cb) and (ck, cbk), so a march through free space,
-// or along a brick, reads no memory.
+// or along a brick, reads no memory.  (A stencil inside a free super-brick whose
+// brick's halo reaches into a neighbour with a lower bound is then evaluated
+// when that brick is the cached one: conservative.)
 __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float ya, float za, float thr, int& cs,
                                          float& cb, int& ck, float& cbk)
 {
     const float hi = (float)a.n - 1.0f - kMargin;
     if (!(xa >= kMargin && ya >= kMargin && za >= kMargin && xa < hi && ya < hi && za < hi)) return -1;
     const int xi = (int)xa, yi = (int)ya, zi = (int)za;
+    // the exact first corner is in brick (bx, by, bz) when the coordinate is
+    // kMargin inside the brick's [0, 8) span on every axis
+    const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
+    const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
+    const float bhi = (float)kBrick - kMargin;
+    const bool in_brick = vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi;
+    const int bidx = grid_index(bx, by, bz, a.nb);
+    // the cached brick decides first (along a brick or near a surface, where the
+    // super-brick test rarely passes): no memory read and no super-brick test
+    if (in_brick && bidx == ck) return cbk >= thr ? 1 : 0;
     const int sx = xi >> 6, sy = yi >> 6, sz = zi >> 6;
     const float ux = xa - (float)(sx * kSuper), uy = ya - (float)(sy * kSuper), uz = za - (float)(sz * kSuper);
     const float shi = (float)(kSuper - 1) - kMargin;
@@ -211,17 +223,9 @@ __device__ __forceinline__ int fast_skip(const RaycastVolume& a, float xa, float
         }
         if (cb >= thr) return 1;
     }
-    // the exact first corner is in brick (bx, by, bz) when the coordinate is
-    // kMargin inside the brick's [0, 8) span on every axis
-    const int bx = xi >> 3, by = yi >> 3, bz = zi >> 3;
-    const float vx = xa - (float)(bx * kBrick), vy = ya - (float)(by * kBrick), vz = za - (float)(bz * kBrick);
-    const float bhi = (float)kBrick - kMargin;
-    if (vx >= kMargin && vy >= kMargin && vz >= kMargin && vx < bhi && vy < bhi && vz < bhi) {
-        const int bidx = grid_index(bx, by, bz, a.nb);
-        if (bidx != ck) {
-            ck = bidx;
-            cbk = order_value(a.brick_key[bidx]);
-        }
+    if (in_brick) {
+        ck = bidx;
+        cbk = order_value(a.brick_key[bidx]);
         return cbk >= thr ? 1 : 0;
     }
     return -1;
