@@ -288,7 +288,7 @@ def serving(args, kfusion, shape, batched_fps):
     on every inter-arrival (seed 7), replaying a 30-frame bank ping-pong.  (2) its capacity: the
     same sessions all open at once at 1 kHz each, so every tick is full (latency is then queueing
     time and is not reported).  (3) the headline form: S sessions of 640x480 -> 512^3 snorm16
-    (seed 100 + s) offered Poisson at 85 % of the batched capacity, reporting aggregate frames/s
+    (seed 100 + s) offered Poisson at 80 % of the batched capacity, reporting aggregate frames/s
     with p50/p99 latency over the whole run and over its steady second half."""
     import numpy as np
 
@@ -315,7 +315,7 @@ def serving(args, kfusion, shape, batched_fps):
     tbanks = [q.frames(0, 30) for q in tseqs]
     tcfgs = [kfusion.SessionConfig(W, H, 525.0, 525.0, 319.5, 239.5, 1e-3, 512, L, mue, kfusion.KF_VOXEL_SNORM16_U8,
                                    tseqs[s].pose(0)) for s in range(S)]
-    rate = 0.85 * batched_fps / S if batched_fps else 30.0
+    rate = 0.8 * batched_fps / S if batched_fps else 30.0
     nfr = max(10, int(round(rate * args.serve_target_seconds)))
     srv = kfusion.Server(S, budget, icp=icp, use_graph=True, shape=shape)
     _, tgt = srv.run_load(tcfgs, tbanks, [nfr] * S, np.zeros(S), rate_hz=rate, jitter=0.1, seed=7)
@@ -350,7 +350,7 @@ def serving(args, kfusion, shape, batched_fps):
                                   "mean_frames_per_tick": round(tgt["mean_frames_per_tick"], 2),
                                   "icp_failed_sessions": int(tgt["failed_frames"]),
                                   "load": f"{S} sessions x 640x480 -> 512^3 snorm16+u8 (seeds 100..{99 + S}), all "
-                                          f"open at once, Poisson {rate:.1f} Hz +-10 % per session (85 % of the "
+                                          f"open at once, Poisson {rate:.1f} Hz +-10 % per session (80 % of the "
                                           f"batched capacity), {nfr} frames each; frames_per_s and offered are over "
                                           "the whole run (first arrival to last completion), steady_* over the "
                                           "second half of the completions"}}
