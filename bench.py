@@ -21,7 +21,7 @@ timed region.  `roofline_raycast` is the raycast (first pass + finishing pass,
 voxels the reference march's samples read (counted by the oracle on the model
 after the timed frames).  "batched" and "serving" report the multi-session
 forms (see their docstrings); every input is resident before timing.
-"precision" runs the precision form on config 3's workload and compares it
+"precision" runs the precision form on its 640x480 -> 256^3 workload and compares it
 with the FP64 oracle of the same algorithm.  The oracle runs only in the CPU leg
 at the end (cpu_leg): the CPU baseline, the |S| counts and the FP64 comparison;
 no GPU figure is measured while it runs.
@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-footprint", action="store_true", help="skip the oracle's raycast footprint counts")
     ap.add_argument("--no-precision", action="store_true",
-                    help="skip the precision form's run against the FP64 oracle (config 3's workload)")
+                    help="skip the precision form's run against the FP64 oracle")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (the known-size copy that "
                          "scales HBM counters in counter-collection runs); not part of the JSON line")
@@ -281,7 +281,7 @@ def cpu_baseline(seq, raws, n, L, mue, budget_s):
 
 
 def serving(args, kfusion, shape, batched_fps):
-    """The serving loop.  (1) config 5: sessions s = 0..S-1 (scene seed 1000 + s) of 640x480
+    """The serving loop.  (1) mixed volumes: sessions s = 0..S-1 (scene seed 1000 + s) of 640x480
     depth, volumes drawn from {256, 384, 512, 640, 768}^3 (seed 99) as snorm16 + uint8 over 4 m
     cubes, admitted against a 200 GB HBM budget as they open (uniformly over the first second);
     each sends --serve-frames frames as a Poisson process of mean 30 Hz with +-10 % uniform jitter
@@ -381,7 +381,7 @@ PREC_FRAMES, PREC_N, PREC_L = 30, 256, 3.0
 
 
 def precision_run(kfusion):
-    """The precision form on config 3's workload: seed-1 frames, 640x480 -> 256^3 snorm16 + uint8
+    """The precision form on its own workload: seed-1 frames, 640x480 -> 256^3 snorm16 + uint8
     counts over a 3 m cube, Kahan-compensated ICP sums and FP64 solve, 30 frames.  Returns what the
     CPU leg compares with the FP64 oracle of the same algorithm: every frame's pose, the final
     volume and the predicted vertex maps."""
@@ -463,7 +463,7 @@ def cpu_leg(result, args, seq, raws, n, L, mue, plain_model, batch_models, prec)
         F64, W64 = op.tsdf()
         both = (prec["W"] > 0) & (W64 > 0)
         dF = np.abs(prec["F"].astype(np.float64) - F64)[both]
-        out = {"workload": f"config 3: seed-1 frames, 640x480 -> {PREC_N}^3 snorm16 + uint8 counts over "
+        out = {"workload": f"seed-1 frames, 640x480 -> {PREC_N}^3 snorm16 + uint8 counts over "
                            f"{PREC_L:g} m, Kahan-compensated ICP sums, FP64 solve, {PREC_FRAMES} frames",
                "reference": "FP64 oracle of the same algorithm (count weights, unquantised F), run independently",
                "icp_status": int(prec["status"]),

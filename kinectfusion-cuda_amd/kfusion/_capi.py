@@ -274,8 +274,7 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
-                f"libkfusion.so is not built ({LIB_PATH}); run __graft_entry__.build() or "
-                f"`make -C {PKG_ROOT}` — there is no CPU fallback")
+                f"libkfusion.so is not built ({LIB_PATH}); run `make -C {PKG_ROOT}` — there is no CPU fallback")
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(lib, name)

@@ -38,7 +38,7 @@ def _p(a):
 
 
 class SyntheticSequence:
-    """BASELINE preset scene: seeded trajectory, spheres and noisy depth frames."""
+    """Seeded synthetic scene (a room with spheres): camera trajectory and noisy depth frames."""
 
     def __init__(self, seed=1, n_frames=30, L=3.0, width=640, height=480, fx=525.0, fy=525.0, cx=None, cy=None,
                  noise_k=0.0012, dropout=0.0, n_spheres=8):

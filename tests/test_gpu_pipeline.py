@@ -105,7 +105,7 @@ def test_pipeline_small_compensated(kf, oracle32):
     replay_with_gpu_poses(kf, oracle32, seq, 128, 0.03, raws, gposes, gp)
 
 
-def test_pipeline_config1(kf, oracle32):
+def test_pipeline_640x480_to_256(kf, oracle32):
     """Plain method, small: 30 frames of 640x480, 256^3 FP32 over a 3 m cube, mue 3 cm, seed 1,
     FP32 tree sums: poses within 1e-4 of the independent oracle run; TSDF and predicted
     maps bit-exact against the oracle replayed at the GPU's poses."""
@@ -113,7 +113,7 @@ def test_pipeline_config1(kf, oracle32):
     seq = SyntheticSequence(seed=1, n_frames=30, L=3.0)
     gp, op, gposes, raws, worst = run_pair(kf, seq, 256, 0.03, 30)
     st = independent_stats(gp, op)
-    print("config1 parity:", worst, st, flush=True)
+    print("640x480 -> 256^3 parity:", worst, st, flush=True)
     assert worst["rot"] <= 1e-4 and worst["trans"] <= 1e-4, worst
     for lvl in range(3):
         assert st[f"raycast_L{lvl}_within_1mm"] >= 0.999, st

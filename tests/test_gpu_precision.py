@@ -1,7 +1,7 @@
 """Precision form (snorm16 TSDF + uint8 observation counts, compensated ICP
 sums, FP64 solve): GPU against the oracle's precision mode bit for bit (fusion
 codes, weights, raycast), and errors against the FP64 oracle of the same
-algorithm stated per result (config 3 at reduced size)."""
+algorithm stated per result (the precision workload at reduced size)."""
 import numpy as np
 import pytest
 
@@ -69,12 +69,12 @@ def test_weight_cap_128(kf, oracle32):
 
 
 def test_precision_pipeline_vs_fp64_oracle(kf):
-    """Config-3 style run at reduced size (320x240, 128^3, 10 frames): the precision-form GPU
+    """The precision workload at reduced size (320x240, 128^3, 10 frames): the precision-form GPU
     pipeline against the FP64 oracle of the same algorithm (count weights on unquantised F), so the
     difference is the snorm16 rounding and the FP32 arithmetic only: poses within 1e-4 rad / 1e-4 m
     and raycast depth within 1 mm on >= 99.9 % of the pixels.  The FP64 oracle of the plain
     algorithm (1/d weights) is reported beside it: the two weight rules alone differ by ~1.8e-4 rad
-    here.  (The full-size case is tests/test_gpu_configs.py::test_config3_precision_vs_fp64.)"""
+    here.  (The full-size case is tests/test_gpu_workloads.py::test_precision_640x480_to_256_vs_fp64.)"""
     from oracle.kfref import OraclePipeline
     seq = small_sequence(seed=1, n_frames=10, w=320, h=240, L=3.0)
     raws = seq.frames()

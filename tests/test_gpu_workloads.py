@@ -1,21 +1,21 @@
-"""Oracle parity at the sizes of the five benchmark configurations (the per-frame
+"""Oracle parity at the sizes of the benchmark workloads (the per-frame
 loop, project/src/kinect_fusion.cpp:44-183, with tsdf.cu:20-75, raycast.cu:44-145
 and icp.cpp:25-87 inside it).
 
-Each case runs the GPU path at the configuration's own volume and frame sizes,
+Each case runs the GPU path at the workload's own volume and frame sizes,
 an independent oracle run on the same frames (poses within 1e-4 rad / 1e-4 m),
 and an oracle replay at the poses the GPU committed: fusion (F and W, or the
 snorm16 codes and uint8 counts), the touched-voxel counts and the predicted
 vertex/normal maps must then be bit-exact, because fusion and raycasting are
 exact restatements and only the poses could differ.
 
- - config 2: plain FP32, 640x480 -> 512^3 over 4 m, seed 2;
- - config 3: precision form (snorm16 + uint8 counts, Kahan FP32 ICP sums, FP64
+ - plain FP32, 640x480 -> 512^3 over 4 m, seed 2;
+ - precision form (snorm16 + uint8 counts, Kahan FP32 ICP sums, FP64
    solve), 640x480 -> 256^3 over 3 m, 30 frames, seed 1, with its errors
    against the FP64 oracle of the same algorithm stated per result;
- - config 4: 16 batched sessions of uneven volume (128..512)^3 and frame size
+ - 16 batched sessions of uneven volume (128..512)^3 and frame size
    (320x240, 640x480), one hipGraph per tick, autotuned launch shapes;
- - config 5: served sessions with volumes 256^3..768^3 under a Poisson load.
+ - served sessions with volumes 256^3..768^3 under a Poisson load.
 """
 import numpy as np
 import pytest
@@ -62,31 +62,31 @@ def run_pipeline(kf, seq, raws, n, mue, fmt, compensated):
     return poses, touched, F, W, maps
 
 
-def test_config2_plain_640x480_512(kf, oracle32):
-    """Config 2 at its own size: 640x480 -> 512^3 FP32 f + FP32 w over a 4 m cube, mue 3 cm, seed 2
+def test_plain_640x480_to_512_fp32(kf, oracle32):
+    """The plain workload at its own size: 640x480 -> 512^3 FP32 f + FP32 w over a 4 m cube, mue 3 cm, seed 2
     (the 300-frame sequence), the first 5 frames.  At 512^3 the brick cull's margins (bounding
     radius, occlusion slack, pixel box) are tightest relative to a voxel."""
     n, L, mue, frames = 512, 4.0, 0.03, 5
     seq = sequence(2, 300, L)
     raws = seq.frames(0, frames)
     poses, touched, F, W, maps = run_pipeline(kf, seq, raws, n, mue, kf.KF_VOXEL_F32, compensated=False)
-    announce("config 2: independent oracle run")
+    announce("plain 512^3: independent oracle run")
     op, oposes = independent(seq, raws, n, mue)
     op.close()
     worst = [pose_diff(a, b) for a, b in zip(poses, oposes)]
-    print("config 2 per-frame pose differences (rad, m):", worst, flush=True)
+    print("plain 512^3 per-frame pose differences (rad, m):", worst, flush=True)
     assert max(r for r, _ in worst) <= 1e-4 and max(t for _, t in worst) <= 1e-4
-    announce("config 2: oracle replay at the GPU's poses")
+    announce("plain 512^3: oracle replay at the GPU's poses")
     Fo, Wo, otouched, omaps = replay(oracle32, seq, n, mue, raws, poses, snorm16=False)
-    print("config 2 touched voxels per frame:", touched, flush=True)
+    print("plain 512^3 touched voxels per frame:", touched, flush=True)
     assert touched == otouched and min(touched) > 1_000_000
     np.testing.assert_array_equal(W.view(np.uint32), Wo.view(np.uint32))
     np.testing.assert_array_equal(F.view(np.uint32), Fo.view(np.uint32))
-    assert_maps_equal(maps, omaps, "config 2")
+    assert_maps_equal(maps, omaps, "plain 512^3")
 
 
-def test_config3_precision_vs_fp64(kf, oracle32):
-    """Config 3: config 1's frames (seed 1, 640x480, 256^3 over 3 m, 30 frames) in the precision
+def test_precision_640x480_to_256_vs_fp64(kf, oracle32):
+    """The precision workload: the seed-1 frames (640x480, 256^3 over 3 m, 30 frames) in the precision
     form.  Bit-exact: snorm16 codes, uint8 counts, touched counts and predicted maps against the
     oracle's precision mode replayed at the GPU's poses.  Against the FP64 oracle of the same
     algorithm (count weights, unquantised F): every frame's pose within 1e-4 rad / 1e-4 m, raycast
@@ -97,14 +97,14 @@ def test_config3_precision_vs_fp64(kf, oracle32):
     seq = sequence(1, 30, L)
     raws = seq.frames(0, frames)
     poses, touched, F, W, maps = run_pipeline(kf, seq, raws, n, mue, kf.KF_VOXEL_SNORM16_U8, compensated=True)
-    announce("config 3: oracle replay (precision mode) at the GPU's poses")
+    announce("precision 256^3: oracle replay (precision mode) at the GPU's poses")
     Fo, Wo, otouched, omaps = replay(oracle32, seq, n, mue, raws, poses, snorm16=True)
     assert touched == otouched
     np.testing.assert_array_equal(W, Wo)
     np.testing.assert_array_equal(F.view(np.uint32), Fo.view(np.uint32))
-    assert_maps_equal(maps, omaps, "config 3")
+    assert_maps_equal(maps, omaps, "precision 256^3")
 
-    announce("config 3: FP64 oracle of the precision algorithm")
+    announce("precision 256^3: FP64 oracle of the precision algorithm")
     from oracle.kfref import Oracle
     op64, p64 = independent(seq, raws, n, mue, dtype=np.float64, count_weights=True)
     errs = [pose_diff(a, b) for a, b in zip(poses, p64)]
@@ -141,7 +141,7 @@ def test_config3_precision_vs_fp64(kf, oracle32):
         res[f"tsdf_rms_{name}"] = float(np.sqrt(np.mean(dFr ** 2)))
         res[f"tsdf_within_1e3_{name}"] = float((dFr <= 1e-3).mean())
         res[f"weights_equal_{name}"] = float((Wa.astype(np.float64) == Wr)[seen].mean())
-    print("config 3 precision form vs FP64 oracle:", res, flush=True)
+    print("precision 256^3 vs FP64 oracle:", res, flush=True)
     assert res["max_rot_rad"] <= 1e-4 and res["max_trans_m"] <= 1e-4, res
     for lvl in range(3):
         assert res[f"raycast_L{lvl}_within_1mm"] >= 0.999, res
@@ -159,7 +159,7 @@ def digest(a):
     return hashlib.blake2b(np.ascontiguousarray(a).data, digest_size=20).hexdigest()
 
 
-def config4_sessions():
+def uneven_sessions():
     """16 sessions: volumes from {128, 256, 384, 512}^3 and frames from {320x240, 640x480},
     drawn with seed 99; session s has scene seed 100 + s; 4 m cubes, mue 3 cm, snorm16."""
     rng = np.random.RandomState(99)
@@ -168,13 +168,13 @@ def config4_sessions():
     return [dict(n=int(sizes[s]), w=320 << int(res[s]), h=240 << int(res[s]), seed=100 + s) for s in range(16)]
 
 
-def test_config4_batch16_uneven(kf, oracle32):
-    """Config 4: 16 uneven sessions in one batch (one persistent launch per stage, each tick one
+def test_batch16_uneven_128_to_512(kf, oracle32):
+    """16 uneven sessions in one batch (one persistent launch per stage, each tick one
     hipGraph launch, autotuned launch shapes), 5 ticks.  A 512^3 / 640x480 session and a small one
     are replayed on the oracle (poses against an independent oracle run, volume codes, counts and
     predicted maps bit-exact at the batch's poses); the large one also runs alone in a batch and
     must equal its result in the mix bit for bit."""
-    specs = config4_sessions()
+    specs = uneven_sessions()
     ticks, L, mue = 5, 4.0, 0.03
     seqs = [sequence(sp["seed"], ticks, L, sp["w"], sp["h"]) for sp in specs]
     raws = [q.frames(0, ticks) for q in seqs]
@@ -188,7 +188,7 @@ def test_config4_batch16_uneven(kf, oracle32):
                for k in idx]
         poses = {k: [] for k in idx}
         for t in range(ticks):
-            announce(f"config 4 tick {t} sessions {idx}")
+            announce(f"batch16 tick {t} sessions {idx}")
             b.tick(ids, [devs[k].ptr.value + t * specs[k]["w"] * specs[k]["h"] * 2 for k in idx])
             if per_tick_poses:
                 b.sync()
@@ -208,10 +208,10 @@ def test_config4_batch16_uneven(kf, oracle32):
     big = next(k for k, sp in enumerate(specs) if sp["n"] == 512 and sp["w"] == 640)
     small = next(k for k, sp in enumerate(specs) if sp["w"] == 320)
     mix, (shape, tuned) = run(list(range(16)), per_tick_poses=True, keep={big, small})
-    print("config 4 sessions:", [(sp["n"], sp["w"]) for sp in specs], "shape:", shape, "tuned:", tuned, flush=True)
+    print("batch16 sessions:", [(sp["n"], sp["w"]) for sp in specs], "shape:", shape, "tuned:", tuned, flush=True)
     for k in (big, small):
         sp, seq = specs[k], seqs[k]
-        announce(f"config 4: oracle checks of session {k} ({sp['n']}^3, {sp['w']}x{sp['h']})")
+        announce(f"batch16: oracle checks of session {k} ({sp['n']}^3, {sp['w']}x{sp['h']})")
         op, oposes = independent(seq, raws[k], sp["n"], mue, snorm16=True)
         op.close()
         worst = [pose_diff(a, b) for a, b in zip(mix[k]["poses"], oposes)]
@@ -219,18 +219,18 @@ def test_config4_batch16_uneven(kf, oracle32):
         Fo, Wo, _, omaps = replay(oracle32, seq, sp["n"], mue, raws[k], mix[k]["poses"], snorm16=True)
         np.testing.assert_array_equal(mix[k]["vol"][1], Wo)
         np.testing.assert_array_equal(mix[k]["vol"][0].view(np.uint32), Fo.view(np.uint32))
-        assert_maps_equal(mix[k]["maps"], omaps, f"config 4 session {k}")
+        assert_maps_equal(mix[k]["maps"], omaps, f"batch16 session {k}")
     alone, _ = run([big], per_tick_poses=False, keep={big})
     np.testing.assert_array_equal(alone[big]["pose"], mix[big]["pose"])
     np.testing.assert_array_equal(alone[big]["vol"][0], mix[big]["vol"][0])
     np.testing.assert_array_equal(alone[big]["vol"][1], mix[big]["vol"][1])
-    assert_maps_equal(alone[big]["maps"], mix[big]["maps"], "config 4 alone vs mix")
+    assert_maps_equal(alone[big]["maps"], mix[big]["maps"], "batch16 alone vs mix")
     for d in devs:
         d.free()
 
 
-def test_config5_served_256_to_768(kf, oracle32):
-    """Config 5: served sessions of 640x480 depth with volumes 256^3..768^3 (snorm16 + uint8, 4 m
+def test_served_256_to_768(kf, oracle32):
+    """Served sessions of 640x480 depth with volumes 256^3..768^3 (snorm16 + uint8, 4 m
     cubes), Poisson arrivals (30 Hz +-10 %, seed 7) under a 200 GB budget, every volume resident.
     The 256^3 session equals the per-frame pipeline on the frames it received (final pose and volume
     bit for bit), whose poses the oracle replays to the same volume; every other session's result is
@@ -246,7 +246,7 @@ def test_config5_served_256_to_768(kf, oracle32):
     def serve(idx):
         srv = kf.Server(len(idx), 200 * 10 ** 9, icp=kf.IcpConfig(ITERS, 640, 480, 0.1, TH, compensated=True),
                         use_graph=True)
-        announce(f"config 5 serving sessions {idx}")
+        announce(f"served 256..768 sessions {idx}")
         tickets, st = srv.run_load([cfgs[k] for k in idx], [banks[k] for k in idx], [frames] * len(idx),
                                    [0.005 * q for q in range(len(idx))], rate_hz=30.0, jitter=0.1, seed=7,
                                    keep_open=True)
@@ -259,7 +259,7 @@ def test_config5_served_256_to_768(kf, oracle32):
         return out, st
 
     mix, st = serve(list(range(len(sizes))))
-    print("config 5 mix:", {k: st[k] for k in ("frames", "ticks", "mean_frames_per_tick", "latency_p50_ms",
+    print("served 256..768 mix:", {k: st[k] for k in ("frames", "ticks", "mean_frames_per_tick", "latency_p50_ms",
                                                "latency_p99_ms", "peak_resident_bytes")}, flush=True)
     # the 256^3 session against the oracle, through the per-frame pipeline on the same frames
     seq, raws = seqs[0], banks[0][order]
@@ -267,7 +267,7 @@ def test_config5_served_256_to_768(kf, oracle32):
     np.testing.assert_array_equal(mix[0][0], poses[-1])
     np.testing.assert_array_equal(mix[0][1][0], F)
     np.testing.assert_array_equal(mix[0][1][1], W)
-    announce("config 5: oracle checks of the 256^3 session")
+    announce("served 256..768: oracle checks of the 256^3 session")
     op, oposes = independent(seq, raws, 256, mue, snorm16=True)
     op.close()
     worst = [pose_diff(a, b) for a, b in zip(poses, oposes)]
