@@ -483,7 +483,7 @@ def cpu_leg(result, args, seq, raws, n, L, mue, plain_model, batch_models, prec)
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "the path runs on one GPU: --gpus must be 1",
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
     import ctypes as C
