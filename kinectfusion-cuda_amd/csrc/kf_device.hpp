@@ -76,7 +76,9 @@ __device__ __forceinline__ v3 m4p(const float* m, v3 p)
 }
 
 __device__ __forceinline__ bool is_depth_valid(float d) { return d > kMinDepth && isfinite(d); }
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped to [lo, hi]; lo <= hi (every caller clamps to [0, size - 1] of a
+// non-empty range), which lets it be a max and a min (no compare and select)
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // float4 map access; out-of-range validity reads are invalid (device_helper.cuh:43-50)
 __device__ __forceinline__ bool map_valid(const float4* m, int w, int h, int i, int j)

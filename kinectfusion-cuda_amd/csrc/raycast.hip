@@ -105,10 +105,12 @@ __device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
     st.a = x - fx;
     st.b = y - fy;
     st.c = z - fz;
+    // sample positions are finite, so the clamp to [-1, n] (which keeps the
+    // conversion in range) is a max and a min
     const float lo = -1.0f, hi = (float)n;
-    fx = fx < lo ? lo : (fx > hi ? hi : fx);
-    fy = fy < lo ? lo : (fy > hi ? hi : fy);
-    fz = fz < lo ? lo : (fz > hi ? hi : fz);
+    fx = fminf(fmaxf(fx, lo), hi);
+    fy = fminf(fmaxf(fy, lo), hi);
+    fz = fminf(fmaxf(fz, lo), hi);
     int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
     st.i1 = clampi(i0 + 1, 0, n - 1);
     st.j1 = clampi(j0 + 1, 0, n - 1);
