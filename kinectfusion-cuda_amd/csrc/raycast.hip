@@ -559,9 +559,38 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
     write_surface<FMT>(a, L, i, j, g, tstar);
 }
 
+// Position of sample `lane` of a chunk of 64: t_0 = t and t_{k+1} = t_k + step,
+// one rounded addition per sample as the march takes them.  The additions form
+// a chain, but inside one binade of t they nearly always add the same rounded
+// increment: each lane guesses t_{k-1} on the line t_b + (k - 1 - b) d through
+// an exact t_b, takes one rounded step from the guess, and checks that the
+// result is the line's next point (so the next lane's guess is exact).  The
+// first lane whose check fails still got the exact value (its own guess was
+// exact) and becomes the next t_b.  By induction every lane ends with the
+// chained value; a chunk takes one pass per change of the increment (a binade
+// crossing, the settling of a rounding tie) instead of 63 dependent additions.
+__device__ __forceinline__ float chunk_position(float t, float step, int lane)
+{
+    float res = t;
+    int b = 0;
+    float tb = t;
+    for (;;) {
+        const float d = (tb + step) - tb;
+        const float c = fmaf((float)(lane - b - 1), d, tb);  // t_{lane-1} when the guesses so far held
+        const float tk = c + step;
+        const bool on = tk == fmaf((float)(lane - b), d, tb);
+        if (lane > b) res = tk;
+        const uint64_t off = __ballot(lane > b && !on);
+        if (!off) break;
+        b = __ffsll((unsigned long long)off) - 1;
+        tb = lane_bcast(res, b);
+    }
+    return res;
+}
+
 // The finishing pass for one handed-over ray: the wave marches it 64 samples
 // at a time, lane k taking sample k of the chunk (its position by the same
-// repeated additions t = t + step), and a ballot finds the first sample that
+// repeated additions t = t + step, chunk_position), and a ballot finds the first sample that
 // ends the ray or changes the march (end of the interval or the guard, f < 0,
 // or f < 0.99 on a large step).  The samples before it are consumed as the
 // single-lane march would (prev from the last of them), the ones after it are
@@ -586,8 +615,7 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
     CellCache cc;
     float tstar = 0.0f;
     for (;;) {
-        float tk = t;
-        for (int q = 0; q < lane; ++q) tk = tk + step;
+        const float tk = chunk_position(t, step, lane);
         const bool end = !(tk < g.farv) || gb + lane > kMaxMarchSteps;
         float f = 0.0f;
         int sk = 1;
