@@ -75,6 +75,9 @@ struct FuseArgs {
     int* super_key;
     int ns;
     int cull;     // 1 when K is a plain pinhole matrix (culling planes valid)
+    // lengths of the culling planes' normals |(fx, cx)|, |(fx, w - cx)|, |(fy, cy)|,
+    // |(fy, h - cy)|, rounded up (they scale the bounding-sphere radius)
+    float plane_norm[4];
     int format;   // kf_voxel_format
     // occlusion test: max valid depth per 16x16 tile (tx16 x ty16) followed by
     // the max per 64x64 tile (tx64 x ty64); null = frustum test only

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <cmath>
 #include <vector>
 
 #include "kf_device.hpp"
@@ -48,22 +49,27 @@ __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip
     float fx = a.K[0], fy = a.K[4], px = a.K[6], py = a.K[7];
     float W = (float)a.w, H = (float)a.h;
     // pixel.x >= 0  <=>  fx*x + px*z >= 0 (z > 0);  pixel.x < W  <=>  (W-px)*z - fx*x > 0
-    if ((fx * ex + px * ez) < -r * sqrtf(fx * fx + px * px)) return false;
-    if (((W - px) * ez - fx * ex) < -r * sqrtf(fx * fx + (W - px) * (W - px))) return false;
-    if ((fy * ey + py * ez) < -r * sqrtf(fy * fy + py * py)) return false;
-    if (((H - py) * ez - fy * ey) < -r * sqrtf(fy * fy + (H - py) * (H - py))) return false;
+    if ((fx * ex + px * ez) < -r * a.plane_norm[0]) return false;
+    if (((W - px) * ez - fx * ex) < -r * a.plane_norm[1]) return false;
+    if ((fy * ey + py * ez) < -r * a.plane_norm[2]) return false;
+    if (((H - py) * ez - fy * ey) < -r * a.plane_norm[3]) return false;
     // Occlusion: a voxel is updated only if d - z >= -mue for the valid depth d
     // of its pixel.  With the sphere beyond the near plane its pixels lie in the
     // box below (x/z and y/z at the corners, one pixel of margin); if even the
     // largest valid depth there is more than mue (+1 mm for rounding) in front
-    // of the sphere's nearest point, no voxel of the brick is updated.
+    // of the sphere's nearest point, no voxel of the brick is updated.  The
+    // quotients come from the hardware reciprocal: within a few 1e-7 of x/z,
+    // which moves the box by under 1e-2 pixel (|x/z| is at most a few tens with
+    // z > kMinDistance), far inside its one pixel of margin.
     const float zlo = ez - r;
     if (!a.dmax || !(zlo > kMinDistance)) return true;
     const float zhi = ez + r, xa = ex - r, xb = ex + r, ya = ey - r, yb = ey + r;
-    const float u0 = fx * fminf(xa / zlo, xa / zhi) + px - 1.0f, u1 = fx * fmaxf(xb / zlo, xb / zhi) + px + 1.0f;
-    const float v0 = fy * fminf(ya / zlo, ya / zhi) + py - 1.0f, v1 = fy * fmaxf(yb / zlo, yb / zhi) + py + 1.0f;
+    const float r0 = __builtin_amdgcn_rcpf(zlo), r1 = __builtin_amdgcn_rcpf(zhi);
+    const float u0 = fx * fminf(xa * r0, xa * r1) + px - 1.0f, u1 = fx * fmaxf(xb * r0, xb * r1) + px + 1.0f;
+    const float v0 = fy * fminf(ya * r0, ya * r1) + py - 1.0f, v1 = fy * fmaxf(yb * r0, yb * r1) + py + 1.0f;
     const int i0 = (int)fmaxf(u0, 0.0f), i1 = (int)fminf(u1, W - 1.0f);
     const int j0 = (int)fmaxf(v0, 0.0f), j1 = (int)fminf(v1, H - 1.0f);
+    if (i1 < i0 || j1 < j0) return a.mue + 1e-3f >= zlo;  // no pixel: no valid depth
     const float* m;
     int shift, tx;
     if (i1 - i0 < 48 && j1 - j0 < 48) {
@@ -77,9 +83,16 @@ __device__ __forceinline__ bool brick_visible(const FuseArgs& a, const float* ip
     } else {
         return true;
     }
+    // a box under 48 (192) pixels spans at most 4 x 4 tiles of 16 (64): all 16
+    // reads are issued together, those past the box's last tile row or column
+    // repeating it (the maximum is the same)
+    const int ti0 = i0 >> shift, tj0 = j0 >> shift, ti1 = i1 >> shift, tj1 = j1 >> shift;
+    float t[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t[q] = m[min(tj0 + (q >> 2), tj1) * tx + min(ti0 + (q & 3), ti1)];
     float dm = 0.0f;
-    for (int tj = j0 >> shift; tj <= (j1 >> shift); ++tj)
-        for (int ti = i0 >> shift; ti <= (i1 >> shift); ++ti) dm = fmaxf(dm, m[tj * tx + ti]);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dm = fmaxf(dm, t[q]);
     return dm + a.mue + 1e-3f >= zlo;
 }
 
@@ -536,49 +549,72 @@ __global__ __launch_bounds__(64 * kFuseWaves, 7) void fuse_list_kernel(FuseArgs 
 // cull writes list entries job << 32 | pack_brick(bi, bj, bk) (the brick's
 // coordinates in the job's volume), so the sweep needs no search.
 constexpr int kBatchCullThreads = 256;
+// Bricks a lane tests between two appends to the list: each test is a chain of
+// dependent loads (the job search, the session's pose, the depth tile maxima),
+// so a lane runs kBatchCullPer independent chains at once, and the workgroup
+// takes one list slot range (one atomic on the tick's single counter) per
+// kBatchCullPer * kBatchCullThreads bricks.
+constexpr int kBatchCullPer = 4;
+
+// Brick g of the tick: its list entry and whether it is listed.
+__device__ __forceinline__ bool batch_cull_one(const FuseJob* jobs, int njobs, unsigned g, unsigned total,
+                                               unsigned long long& entry)
+{
+    entry = 0;
+    if (g >= total) return false;
+    const int j = job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; });
+    const FuseJob& J = jobs[j];
+    const FuseArgs& a = J.a;
+    if (a.dyn->frame_skip_fuse) return false;
+    const float* ip = a.dyn->inv_pose;
+    const unsigned brick = g - J.brick0, nb = (unsigned)a.nb;
+    for (unsigned k = brick; k < (unsigned)a.n; k += nb * nb * nb) z_terms(ip, a.zk, a.zt, (int)k);
+    const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / (nb * nb));
+    entry = (unsigned long long)j << 32 | (unsigned long long)pack_brick(bi, bj, bk);
+    return !a.cull || brick_visible(a, ip, bi, bj, bk);
+}
 
 __global__ __launch_bounds__(kBatchCullThreads) void fuse_batch_cull_kernel(const BatchHeader* hd,
                                                                            const FuseJob* jobs)
 {
-    __shared__ unsigned int wave_base[kBatchCullThreads / 64 + 1];
+    constexpr int kWaves = kBatchCullThreads / 64;
+    __shared__ unsigned int wave_base[kBatchCullPer * kWaves + 1];
     const unsigned total = hd->fuse_bricks;
     const int njobs = hd->n_fuse;
-    const unsigned stride = gridDim.x * kBatchCullThreads;
+    const unsigned span = kBatchCullThreads * kBatchCullPer;
+    const unsigned stride = gridDim.x * span;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // uniform trip count so every lane reaches the barriers
-    for (unsigned base = blockIdx.x * kBatchCullThreads; base < total; base += stride) {
-        const unsigned g = base + threadIdx.x;
-        bool vis = false;
-        unsigned long long entry = 0;
-        if (g < total) {
-            const int j = job_of(jobs, njobs, g, [](const FuseJob& x) { return x.brick0; });
-            const FuseJob& J = jobs[j];
-            const FuseArgs& a = J.a;
-            if (!a.dyn->frame_skip_fuse) {
-                const float* ip = a.dyn->inv_pose;
-                const unsigned brick = g - J.brick0, nb = (unsigned)a.nb;
-                for (unsigned k = brick; k < (unsigned)a.n; k += nb * nb * nb) z_terms(ip, a.zk, a.zt, (int)k);
-                const int bi = (int)(brick % nb), bj = (int)((brick / nb) % nb), bk = (int)(brick / (nb * nb));
-                entry = (unsigned long long)j << 32 | (unsigned long long)pack_brick(bi, bj, bk);
-                vis = !a.cull || brick_visible(a, ip, bi, bj, bk);
-            }
+    for (unsigned base = blockIdx.x * span; base < total; base += stride) {
+        bool vis[kBatchCullPer];
+        unsigned long long entry[kBatchCullPer];
+        uint64_t m[kBatchCullPer];
+#pragma unroll
+        for (int q = 0; q < kBatchCullPer; ++q)
+            vis[q] = batch_cull_one(jobs, njobs, base + q * kBatchCullThreads + threadIdx.x, total, entry[q]);
+#pragma unroll
+        for (int q = 0; q < kBatchCullPer; ++q) {
+            m[q] = __ballot(vis[q]);
+            if (lane == 0) wave_base[q * kWaves + wave] = (unsigned int)__popcll(m[q]);
         }
-        const uint64_t m = __ballot(vis);
-        if (lane == 0) wave_base[wave] = (unsigned int)__popcll(m);
         __syncthreads();
         if (threadIdx.x == 0) {
+            // slots in brick order: q-th group of kBatchCullThreads bricks, then wave
             unsigned int t = 0;
-            for (int w = 0; w < kBatchCullThreads / 64; ++w) {
+            for (int w = 0; w < kBatchCullPer * kWaves; ++w) {
                 unsigned int c = wave_base[w];
                 wave_base[w] = t;
                 t += c;
             }
-            wave_base[kBatchCullThreads / 64] = t ? atomicAdd(hd->fuse_count, t) : 0u;
+            wave_base[kBatchCullPer * kWaves] = t ? atomicAdd(hd->fuse_count, t) : 0u;
         }
         __syncthreads();
-        if (vis)
-            hd->fuse_list[wave_base[kBatchCullThreads / 64] + wave_base[wave] +
-                          (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = entry;
+        const unsigned int slot0 = wave_base[kBatchCullPer * kWaves];
+#pragma unroll
+        for (int q = 0; q < kBatchCullPer; ++q)
+            if (vis[q])
+                hd->fuse_list[slot0 + wave_base[q * kWaves + wave] + (unsigned int)__popcll(m[q] & ((1ull << lane) - 1ull))] =
+                    entry[q];
         __syncthreads();
     }
 }
@@ -849,6 +885,10 @@ FuseArgs fuse_args(const float* depth, int w, int h, const kf_volume* v, const f
     a.super_key = v->super_key;
     a.ns = v->ns;
     a.cull = is_pinhole(K) ? 1 : 0;
+    const double W = (double)w, H = (double)h;
+    const double pn[4] = {std::hypot((double)K[0], (double)K[6]), std::hypot((double)K[0], W - (double)K[6]),
+                          std::hypot((double)K[4], (double)K[7]), std::hypot((double)K[4], H - (double)K[7])};
+    for (int q = 0; q < 4; ++q) a.plane_norm[q] = std::nextafter((float)pn[q], INFINITY);
     a.format = v->format;
     a.dmax = nullptr;
     a.tx16 = (w + 15) / 16;
