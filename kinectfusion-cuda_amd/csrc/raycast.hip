@@ -706,8 +706,9 @@ __device__ __forceinline__ void record_order(const RaycastArgs& a, int l, int x,
     }
 }
 
-// Five waves per SIMD: 96 VGPRs (from 105) and 20 bytes of spill per lane for the FP32 form;
-// measured 1 % faster than four. The batch form stays at four (five made it 2 % slower).
+// At least five waves per SIMD: the FP32 form compiles to 76 VGPRs (six waves, no spill).
+// Bounds of seven (68 VGPRs) measured the same and eight (64 VGPRs, 20 bytes of spill per
+// lane) 3.7 % slower per frame.
 template <int FMT>
 __global__ __launch_bounds__(64, 5) void raycast_kernel(RaycastArgs a)
 {
