@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kCandidates = 4;
 constexpr int kTuneReps = 3;
-constexpr unsigned kMaxFuseBlocks = 8192;  // touched slots
+constexpr unsigned kMaxFuseBlocks = 32768;  // touched slots
 
 struct Session {
     bool used = false;
@@ -65,10 +65,10 @@ struct Slot {
 };
 
 const unsigned kCand[4][kCandidates] = {
-    {512, 1024, 2048, 4096},     // preprocess
+    {2048, 4096, 8192, 16384},   // preprocess
     {2048, 4096, 8192, 16384},   // raycast
     {256, 512, 1024, 2048},      // ICP steps
-    {1024, 2048, 4096, 8192},    // fuse list
+    {4096, 8192, 16384, 32768},  // fuse list
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -366,11 +366,12 @@ void finish_tuning(kf_batch* b)
 
 void default_shape(kf_batch_shape& s, const kf_batch_shape& given)
 {
-    s.preprocess_blocks = given.preprocess_blocks ? given.preprocess_blocks : 2048;
+    // the shapes the tuner picks for 64 sessions of 640x480 -> 512^3
+    s.preprocess_blocks = given.preprocess_blocks ? given.preprocess_blocks : 8192;
     s.raycast_blocks = given.raycast_blocks ? given.raycast_blocks : 8192;
     s.icp_blocks = given.icp_blocks ? given.icp_blocks : 1024;
     s.cull_blocks = given.cull_blocks ? given.cull_blocks : 4096;  // 0.31 ms per 64 x 512^3 tick at 1024, 0.18 at 4096
-    s.fuse_blocks = given.fuse_blocks ? given.fuse_blocks : 2048;
+    s.fuse_blocks = given.fuse_blocks ? given.fuse_blocks : 16384;
 }
 
 bool valid_shape(const kf_batch_shape& s)
