@@ -129,7 +129,9 @@ __device__ __forceinline__ float kf_expf(float x)
 }
 // kf_expf for x <= 0 or NaN (the bilateral weights, -a - b with a, b >= 0):
 // the overflow branches of kf_expf are never taken there, so the same
-// operations without them give the same bits.
+// operations without them give the same bits.  Its scaling by 2^k is one
+// rounding of y * 2^k (for k < -126 the first step, by 2^-64, is exact, and
+// y in [0.7, 1.5] keeps it normal), which is what v_ldexp_f32 computes.
 __device__ __forceinline__ float kf_expf_nonpos(float x)
 {
     if (!(x == x)) return x;
@@ -147,8 +149,7 @@ __device__ __forceinline__ float kf_expf_nonpos(float x)
     p = fmaf(p, r, 5.0000001201e-1f);
     float y = fmaf(p, z, r);
     y = y + 1.0f;
-    if (k < -126) { y = y * 5.421010862427522e-20f; k += 64; }
-    return y * __int_as_float((k + 127) << 23);
+    return __builtin_amdgcn_ldexpf(y, k);
 }
 
 // Live normal at (i,j) of a vertex map (device_helper.cuh:81-110).

@@ -33,23 +33,26 @@ __global__ __launch_bounds__(256) void convert_kernel(const uint16_t* __restrict
 // thread's offset.
 typedef float BilateralTile[kTile + 6][kTile + 6 + 1];
 
+// The weight's argument is -sd / sigma_s^2 - id / sigma_r^2 with
+// 1 / sigma_s^2 = 1/16 and 1 / sigma_r^2 = 16: both products are exact (powers
+// of two; 16 id overflows exactly when the product does), so it is one
+// rounding of c - 16 id with c = -(x^2 + y^2) / 16 exact, i.e. one fused
+// multiply-add on a per-tap constant.
 __device__ __forceinline__ float bilateral_at(const BilateralTile& tile, int tx, int ty)
 {
-    const float inv_s = 1.0f / (4.0f * 4.0f);
-    const float inv_r = 1.0f / (0.25f * 0.25f);
     float center = tile[ty + 3][tx + 3];
     float norm = 0.0f, acc = 0.0f;
     // the outer loop stays rolled: unrolled, all 49 taps' tile reads were held
     // at once (106 VGPRs, 4 waves per SIMD); rolled, 27 VGPRs and 8 waves
 #pragma unroll 1
     for (int x = -3; x <= 3; ++x) {
+        const float cx = (float)(x * x) * -0.0625f;
 #pragma unroll
         for (int y = -3; y <= 3; ++y) {
             float cur = tile[ty + 3 + y][tx + 3 + x];
-            float sd = (float)(x * x + y * y);
             float id = center - cur;
             id = id * id;
-            float f = kf_expf_nonpos(-sd * inv_s - id * inv_r);
+            float f = kf_expf_nonpos(fmaf(id, -16.0f, cx + (float)(y * y) * -0.0625f));
             norm = norm + f;
             acc = acc + f * cur;
         }

@@ -111,6 +111,55 @@ def test_preprocess_fused_matches_stages(kf, oracle32):
         np.testing.assert_array_equal(npm[l].download().view(np.uint32), n.view(np.uint32))
 
 
+def _subnormal_weight_taps(d):
+    """(pixel, tap) pairs whose weight exp(-sd/16 - 16 id) is subnormal (argument in [-104, -87.34])."""
+    h, w = d.shape
+    n = 0
+    for x in range(-3, 4):
+        for y in range(-3, 4):
+            nb = d[np.clip(np.arange(h) + y, 0, h - 1)][:, np.clip(np.arange(w) + x, 0, w - 1)]
+            a = -(x * x + y * y) / 16.0 - 16.0 * (d.astype(np.float64) - nb) ** 2
+            n += int(((a >= -104.0) & (a < -87.34)).sum())
+    return n
+
+
+def test_bilateral_subnormal_weights_bit_exact(kf, oracle32):
+    """Depth steps of 2.34-2.55 m put weights in the subnormal range (the exp's
+    arguments in [-104, -87.34]), where the final scaling by 2^k rounds."""
+    rng = np.random.default_rng(11)
+    d = (0.6 + 0.01 * rng.random((96, 128))).astype(np.float32)
+    step = rng.random((96, 128)) < 0.3
+    d[step] += rng.uniform(2.34, 2.55, size=int(step.sum())).astype(np.float32)
+    assert _subnormal_weight_taps(d) > 1000
+    announce("bilateral with subnormal weights: 128x96 random depth steps of 2.34-2.55 m")
+    a, b = kf.GridMap(128, 96), kf.GridMap(128, 96)
+    a.upload(d)
+    kf.kernel.apply_bilateral_filter(a, b)
+    np.testing.assert_array_equal(b.download().view(np.uint32), oracle32.apply_bilateral_filter(d).view(np.uint32))
+    # the fused preprocessing launch: codes 600 and 3000 mm (a 2.4 m step)
+    import ctypes as C
+    raw_h = np.where(rng.random((480, 640)) < 0.3, 3000, 600).astype(np.uint16)
+    r_dm = oracle32.convert_to_depth_meters(raw_h, 1e-3)
+    assert _subnormal_weight_taps(r_dm[:64, :64]) > 1000
+    announce("kf_preprocess with subnormal weights: 640x480 codes 600/3000")
+    raw = kf.DepthMap(640, 480)
+    raw.update(raw_h)
+    dm = kf.GridMap(640, 480)
+    dp = [kf.GridMap(640 >> l, 480 >> l) for l in range(3)]
+    vp = [kf.GridMap(640 >> l, 480 >> l, 4) for l in range(3)]
+    npm = [kf.GridMap(640 >> l, 480 >> l, 4) for l in range(3)]
+    Ks = [oracle32.intrinsics(525.0, 525.0, 319.5, 239.5, l) for l in range(3)]
+    kinv = np.ascontiguousarray(np.stack([k[1] for k in Ks]))
+    arr = C.c_void_p * 3
+    st = kf.lib().kf_preprocess(raw.ptr, 640, 480, 1e-3, 0, 3, kinv.ctypes.data_as(C.c_void_p), dm.ptr,
+                                arr(*[x.ptr.value for x in dp]), arr(*[x.ptr.value for x in vp]),
+                                arr(*[x.ptr.value for x in npm]), None, None)
+    kf.check(st, "kf_preprocess")
+    kf.check(kf.lib().kf_device_synchronize())
+    np.testing.assert_array_equal(dp[0].download().view(np.uint32),
+                                  oracle32.apply_bilateral_filter(r_dm).view(np.uint32))
+
+
 def test_dimension_errors(kf):
     announce("dimension checks (no kernel launch)")
     a, b = kf.GridMap(8, 8), kf.GridMap(8, 4)
