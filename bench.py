@@ -62,6 +62,8 @@ def parse():
                     help="length of the served 512^3 load offered near capacity")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-footprint", action="store_true", help="skip the oracle's raycast footprint counts")
+    ap.add_argument("--no-early-stop", action="store_true",
+                    help="skip the plain run with the reference application's ICP early stop")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the precision form's run against the FP64 oracle")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -480,6 +482,70 @@ def cpu_leg(result, args, seq, raws, n, L, mue, plain_model, batch_models, prec)
         result["precision"] = out
 
 
+def plain_config(kfusion, seq, n, L, mue, early_stop):
+    """Pipeline configuration of the plain workload (640x480 -> n^3 FP32 over an L cube).  With
+    early_stop, the reference application's ICP stop thresholds (main.cpp:58-59: 0.1 degree, 1 cm;
+    icp.cpp:45-53 moves to the next level once an update changes the pose by less than both)."""
+    import numpy as np
+
+    from kfusion import _capi
+    cfg = _capi.kf_pipeline_config()
+    cfg.frame_width, cfg.frame_height, cfg.depth_scale = W, H, 1e-3
+    cfg.fx, cfg.fy, cfg.cx, cfg.cy = 525.0, 525.0, 319.5, 239.5
+    cfg.volume_n, cfg.volume_size, cfg.mue, cfg.voxel_format = n, L, mue, kfusion.KF_VOXEL_F32
+    for i, v in enumerate(seq.pose(0)):
+        cfg.initial_pose[i] = float(v)
+    cfg.icp = kfusion.IcpConfig((10, 5, 4), W, H, 0.1, float(np.deg2rad(20.0)),
+                                iteration_stop_thresh_angle=float(np.deg2rad(0.1)) if early_stop else 0.0,
+                                iteration_stop_thresh_distance=0.01 if early_stop else 0.0)
+    return cfg
+
+
+def plain_early_stop(args, kfusion, stream, dev, seq, n, L, mue):
+    """The plain workload again with the reference application's ICP early stop: the same frames,
+    warm-up and timed count, in a pipeline of its own.  Levels end early once converged, and the
+    iteration launches after the stop find the level done and return at once (the loop stays on
+    the device), so ICP costs less.  Reported beside `value`, which runs every iteration."""
+    import ctypes as C
+
+    import numpy as np
+    lib = kfusion.lib()
+    cfg = plain_config(kfusion, seq, n, L, mue, True)
+    h = C.c_void_p()
+    kfusion.check(lib.kf_pipeline_create(C.byref(cfg), stream, C.byref(h)), "pipeline")
+    fb = W * H * 2
+    st = C.c_int()
+    for f in range(1 + args.warmup):
+        kfusion.check(lib.kf_pipeline_frame(h, dev.ptr.value + f * fb), "frame")
+    kfusion.check(lib.kf_pipeline_sync(h, C.byref(st)), "sync")
+    kfusion.check(lib.kf_pipeline_set_timing(h, 1), "timing")
+    ev = Events(kfusion)
+    lib.kf_event_record(ev.a, stream)
+    for f in range(1 + args.warmup, 1 + args.warmup + args.steps):
+        kfusion.check(lib.kf_pipeline_frame(h, dev.ptr.value + f * fb), "frame")
+    lib.kf_event_record(ev.b, stream)
+    kfusion.check(lib.kf_pipeline_sync(h, C.byref(st)), "sync")
+    gpu_ms = ev.ms()
+    ev.close()
+    stages = np.zeros(4, np.float32)
+    nfr = C.c_int64()
+    lib.kf_pipeline_stage_times(h, stages.ctypes.data_as(C.c_void_p), C.byref(nfr))
+    pose = np.zeros(16, np.float32)
+    lib.kf_pipeline_pose(h, pose.ctypes.data_as(C.c_void_p))
+    gt = seq.pose(args.warmup + args.steps)
+    lib.kf_pipeline_destroy(h)
+    if st.value != 0:
+        raise RuntimeError(f"ICP failure during the early-stop frames: {st.value}")
+    return {"value": round(args.steps / (gpu_ms / 1e3), 3), "unit": "frames/s", "frames_timed": args.steps,
+            "ms_per_frame": round(gpu_ms / args.steps, 4),
+            "icp": "up to 10/5/4 iterations, a level ends once an update moves the pose by < 0.1 degree and "
+                   "< 1 cm (main.cpp:58-59, icp.cpp:45-53)",
+            "stage_ms_per_frame": {k: round(float(v) / max(1, nfr.value), 4)
+                                   for k, v in zip(kfusion.STAGE_NAMES, stages)},
+            "final_pose_err_vs_truth": {"trans_m": round(float(np.abs(pose[12:15] - gt[12:15]).max()), 5),
+                                        "rot_deg": round(rot_err_deg(pose, gt), 4)}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -491,7 +557,6 @@ def main():
     import numpy as np
 
     import kfusion
-    from kfusion import _capi
     from kfusion.synth import SyntheticSequence
 
     kfusion.init(0)
@@ -503,13 +568,7 @@ def main():
     dev = kfusion.DeviceArray.from_host(raws)
     stream = C.c_void_p()
     kfusion.check(lib.kf_stream_create(C.byref(stream)), "stream")
-    cfg = _capi.kf_pipeline_config()
-    cfg.frame_width, cfg.frame_height, cfg.depth_scale = W, H, 1e-3
-    cfg.fx, cfg.fy, cfg.cx, cfg.cy = 525.0, 525.0, 319.5, 239.5
-    cfg.volume_n, cfg.volume_size, cfg.mue, cfg.voxel_format = n, L, mue, kfusion.KF_VOXEL_F32
-    for i, v in enumerate(seq.pose(0)):
-        cfg.initial_pose[i] = float(v)
-    cfg.icp = kfusion.IcpConfig((10, 5, 4), W, H, 0.1, float(np.deg2rad(20.0)))
+    cfg = plain_config(kfusion, seq, n, L, mue, False)
     h = C.c_void_p()
     kfusion.check(lib.kf_pipeline_create(C.byref(cfg), stream, C.byref(h)), "pipeline")
     frame_bytes = W * H * 2
@@ -588,6 +647,8 @@ def main():
         plain_model = (np.ascontiguousarray(Fv[0::2]), pose.copy(), stage_ms["raycast"])
         del Fv
     lib.kf_pipeline_destroy(h)
+    if not args.no_early_stop:
+        result["plain_reference_early_stop"] = plain_early_stop(args, kfusion, stream, dev, seq, n, L, mue)
     dev.free()
     result["hbm_copy_gbs"] = copy_peak(kfusion, stream)
     result["roofline"]["frac_of_copy"] = round(achieved / result["hbm_copy_gbs"], 4)
