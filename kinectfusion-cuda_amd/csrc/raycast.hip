@@ -748,24 +748,29 @@ __global__ __launch_bounds__(64, 5) void raycast_kernel(RaycastArgs a)
         record_order(a, l, x, per, base, slot, live ? wall_clock64() - t0 : 0ull);
 }
 
-// Workgroups (one wave each) of the finishing pass; each takes every
-// kFinishWaves-th handed-over ray.  The batch's pass (hundreds of thousands of
-// rays per tick) takes every kBatchFinishWaves-th: the rays' lengths vary, and
-// the finer split shortens the tail (per 64-session tick: 0.69 ms at 8192
+// Waves of the finishing pass; each takes every kFinishWaves-th handed-over
+// ray.  The batch's pass (hundreds of thousands of rays per tick) takes every
+// kBatchFinishWaves-th: the rays' lengths vary, and the finer split shortens
+// the tail (per 64-session tick, one wave per workgroup: 0.69 ms at 8192
 // waves, 0.60 at 16384, 0.56 at 32768, 0.54 at 65536).  A single frame's few
-// thousand rays gain nothing from more than 8192.
+// thousand rays gain nothing from more than 8192.  The waves are independent
+// (no LDS, no barrier) and go kFinishPer to a workgroup: fewer workgroups to
+// dispatch took the batch's pass from 0.54 to 0.51 ms (8 per workgroup gave
+// 0.55, 16 gave 0.66 and slowed the single-frame pass).
 constexpr unsigned kFinishWaves = 8192;
 constexpr unsigned kBatchFinishWaves = 65536;
+constexpr unsigned kFinishPer = 4;
 
 template <int FMT>
-__global__ __launch_bounds__(64) void raycast_finish_kernel(RaycastArgs a)
+__global__ __launch_bounds__(64 * kFinishPer) void raycast_finish_kernel(RaycastArgs a)
 {
     typedef __attribute__((address_space(4))) const unsigned ConstCount;
     typedef __attribute__((address_space(4))) const DeferredRay ConstRay;
     const unsigned total = *(ConstCount*)a.count;
     const unsigned n = total < a.cap ? total : a.cap;
     const float* pose = a.dyn ? a.dyn->pose : a.pose;
-    for (unsigned r = blockIdx.x; r < n; r += gridDim.x) {
+    const unsigned w0 = blockIdx.x * kFinishPer + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (unsigned r = w0; r < n; r += gridDim.x * kFinishPer) {
         ConstRay& cr = ((ConstRay*)a.rays)[r];
         DeferredRay d;
         d.job = cr.job;
@@ -862,7 +867,8 @@ __global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd
 // The finishing pass of one voxel format over the rays its first pass handed
 // over (each record names its job).
 template <int FMT>
-__global__ __launch_bounds__(64) void raycast_batch_finish_kernel(const BatchHeader* hd, const RaycastJob* jobs)
+__global__ __launch_bounds__(64 * kFinishPer) void raycast_batch_finish_kernel(const BatchHeader* hd,
+                                                                               const RaycastJob* jobs)
 {
     typedef __attribute__((address_space(4))) const RaycastJob ConstJob;
     typedef __attribute__((address_space(4))) const BatchHeader ConstHeader;
@@ -874,7 +880,8 @@ __global__ __launch_bounds__(64) void raycast_batch_finish_kernel(const BatchHea
     const unsigned total = *(ConstCount*)(ch->ray_next + (FMT == KF_VOXEL_F32 ? 16 : 17));
     const unsigned n = total < cap ? total : cap;
     ConstRay* rays = (ConstRay*)(ch->ray_list + (FMT == KF_VOXEL_F32 ? 0u : cap));
-    for (unsigned r = blockIdx.x; r < n; r += gridDim.x) {
+    const unsigned w0 = blockIdx.x * kFinishPer + (unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (unsigned r = w0; r < n; r += gridDim.x * kFinishPer) {
         DeferredRay d;
         d.job = rays[r].job;
         d.px = rays[r].px;
@@ -918,9 +925,10 @@ int launch_raycast_batch(const BatchHeader* hd, const RaycastJob* jobs, unsigned
     raycast_batch_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_kernel");
     if (mid) KF_HIP(hipEventRecord(mid, s));
-    raycast_batch_finish_kernel<KF_VOXEL_F32><<<kBatchFinishWaves, 64, 0, s>>>(hd, jobs);
+    raycast_batch_finish_kernel<KF_VOXEL_F32><<<kBatchFinishWaves / kFinishPer, 64 * kFinishPer, 0, s>>>(hd, jobs);
     KF_LAUNCHED("raycast_batch_finish_kernel");
-    raycast_batch_finish_kernel<KF_VOXEL_SNORM16_U8><<<kBatchFinishWaves, 64, 0, s>>>(hd, jobs);
+    raycast_batch_finish_kernel<KF_VOXEL_SNORM16_U8><<<kBatchFinishWaves / kFinishPer, 64 * kFinishPer, 0, s>>>(
+        hd, jobs);
     KF_LAUNCHED("raycast_batch_finish_kernel");
     return KF_OK;
 }
@@ -1009,11 +1017,11 @@ int launch_raycast_levels(const kf_volume* v, const float* pose, const PoseState
     if (v->format == KF_VOXEL_F32) {
         raycast_kernel<KF_VOXEL_F32><<<blocks, 64, 0, s>>>(a);
         KF_LAUNCHED("raycast_kernel");
-        raycast_finish_kernel<KF_VOXEL_F32><<<kFinishWaves, 64, 0, s>>>(a);
+        raycast_finish_kernel<KF_VOXEL_F32><<<kFinishWaves / kFinishPer, 64 * kFinishPer, 0, s>>>(a);
     } else {
         raycast_kernel<KF_VOXEL_SNORM16_U8><<<blocks, 64, 0, s>>>(a);
         KF_LAUNCHED("raycast_kernel");
-        raycast_finish_kernel<KF_VOXEL_SNORM16_U8><<<kFinishWaves, 64, 0, s>>>(a);
+        raycast_finish_kernel<KF_VOXEL_SNORM16_U8><<<kFinishWaves / kFinishPer, 64 * kFinishPer, 0, s>>>(a);
     }
     KF_LAUNCHED("raycast_finish_kernel");
     return KF_OK;
