@@ -706,9 +706,9 @@ __device__ __forceinline__ void record_order(const RaycastArgs& a, int l, int x,
     }
 }
 
-// At least five waves per SIMD: the FP32 form compiles to 76 VGPRs (six waves, no spill).
-// Bounds of seven (68 VGPRs) measured the same and eight (64 VGPRs, 20 bytes of spill per
-// lane) made the three-level raycast 4 % slower.
+// At least five waves per SIMD: the FP32 form compiles to 68 VGPRs (seven waves, no spill).
+// A bound of eight (64 VGPRs, 20 bytes of spill per lane) made the three-level raycast 4 %
+// slower.
 template <int FMT>
 __global__ __launch_bounds__(64, 5) void raycast_kernel(RaycastArgs a)
 {
