@@ -97,6 +97,10 @@ struct Stencil {
     int i0, j0, k0, i1, j1, k1;
     float a, b, c;
 };
+// The floors are integers (sample positions are finite), so clamping them, and
+// them plus one (exact below 2^24; beyond, both clamp to the same end), to
+// [0, n-1] in float and converting gives the clamped indices: one median of
+// three and a conversion per index.
 __device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
 {
     Stencil st;
@@ -105,19 +109,13 @@ __device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
     st.a = x - fx;
     st.b = y - fy;
     st.c = z - fz;
-    // sample positions are finite, so the clamp to [-1, n] (which keeps the
-    // conversion in range) is a max and a min
-    const float lo = -1.0f, hi = (float)n;
-    fx = fminf(fmaxf(fx, lo), hi);
-    fy = fminf(fmaxf(fy, lo), hi);
-    fz = fminf(fmaxf(fz, lo), hi);
-    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-    st.i1 = clampi(i0 + 1, 0, n - 1);
-    st.j1 = clampi(j0 + 1, 0, n - 1);
-    st.k1 = clampi(k0 + 1, 0, n - 1);
-    st.i0 = clampi(i0, 0, n - 1);
-    st.j0 = clampi(j0, 0, n - 1);
-    st.k0 = clampi(k0, 0, n - 1);
+    const float top = (float)(n - 1);
+    st.i0 = (int)__builtin_amdgcn_fmed3f(fx, 0.0f, top);
+    st.j0 = (int)__builtin_amdgcn_fmed3f(fy, 0.0f, top);
+    st.k0 = (int)__builtin_amdgcn_fmed3f(fz, 0.0f, top);
+    st.i1 = (int)__builtin_amdgcn_fmed3f(fx + 1.0f, 0.0f, top);
+    st.j1 = (int)__builtin_amdgcn_fmed3f(fy + 1.0f, 0.0f, top);
+    st.k1 = (int)__builtin_amdgcn_fmed3f(fz + 1.0f, 0.0f, top);
     return st;
 }
 
