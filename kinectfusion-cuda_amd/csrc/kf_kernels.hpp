@@ -119,6 +119,7 @@ struct RaycastVolume {
     int n, nb, ns, format;
     float width, mue;
     float inv_width;  // 1 / width when width is a power of two (then x * inv_width == x / width), else 0
+    int p2;           // width and n are powers of two (the march's texel coordinates are then exact)
 };
 struct RaycastLevel {
     float Kinv[9];

@@ -101,10 +101,9 @@ struct Stencil {
 // them plus one (exact below 2^24; beyond, both clamp to the same end), to
 // [0, n-1] in float and converting gives the clamped indices: one median of
 // three and a conversion per index.
-__device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
+__device__ __forceinline__ Stencil stencil_texel(int n, float x, float y, float z)
 {
     Stencil st;
-    float x = u * (float)n - 0.5f, y = v * (float)n - 0.5f, z = s * (float)n - 0.5f;
     float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     st.a = x - fx;
     st.b = y - fy;
@@ -117,6 +116,15 @@ __device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
     st.j1 = (int)__builtin_amdgcn_fmed3f(fy + 1.0f, 0.0f, top);
     st.k1 = (int)__builtin_amdgcn_fmed3f(fz + 1.0f, 0.0f, top);
     return st;
+}
+// From normalized coordinates: x = u*n - 1/2 rounded twice; for a power-of-two
+// n the product is exact, so one fused multiply-add rounds the same way (P2).
+template <bool P2>
+__device__ __forceinline__ Stencil stencil(int n, float u, float v, float s)
+{
+    const float nf = (float)n;
+    if (P2) return stencil_texel(n, fmaf(u, nf, -0.5f), fmaf(v, nf, -0.5f), fmaf(s, nf, -0.5f));
+    return stencil_texel(n, u * nf - 0.5f, v * nf - 0.5f, s * nf - 0.5f);
 }
 
 // Voxel q of the f plane (32-bit element offset from the plane's base).
@@ -162,10 +170,10 @@ __device__ __forceinline__ float trilinear(const void* F, int nb, const Stencil&
     return (1.0f - c) * g0 + c * g1;
 }
 
-template <int FMT>
+template <int FMT, bool P2>
 __device__ __forceinline__ float sample_tsdf(const void* F, int n, int nb, float u, float v, float s)
 {
-    return trilinear<FMT>(F, nb, stencil(n, u, v, s));
+    return trilinear<FMT>(F, nb, stencil<P2>(n, u, v, s));
 }
 
 // Brick holding the stencil's first corner (its halo bound covers the stencil).
@@ -326,18 +334,20 @@ constexpr int kWorkFloor = 16;
 
 // One trilinear evaluation of the march at t: 1 = skipped (the sample's bound
 // is >= thr: kSkipBound on a large step, kSmallSkipBound on a small one),
-// 0 = evaluated into f.
-template <int FMT>
+// 0 = evaluated into f.  With P2 (power-of-two width and n: invw is the exact
+// reciprocal, and tex then x = u*n - 1/2 round like the two fused
+// multiply-adds of approx_texel) the skip test's texel coordinates are the
+// stencil's.
+template <int FMT, bool P2>
 __device__ __forceinline__ int march_sample(const RaycastVolume& a, v3 o, v3 dir, float t, float width, float invw,
                                             float nf, float thr, int& cs, float& cb, int& ck, float& cbk,
                                             CellCache& cc, float& f)
 {
     const v3 p = add3(o, mul3s(dir, t));
-    const int fs = a.brick_key ? fast_skip(a, approx_texel(p.x, invw, nf), approx_texel(p.y, invw, nf),
-                                           approx_texel(p.z, invw, nf), thr, cs, cb, ck, cbk)
-                               : 0;
+    const float xa = approx_texel(p.x, invw, nf), ya = approx_texel(p.y, invw, nf), za = approx_texel(p.z, invw, nf);
+    const int fs = a.brick_key ? fast_skip(a, xa, ya, za, thr, cs, cb, ck, cbk) : 0;
     if (fs == 1) return 1;
-    const Stencil st = stencil(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
+    const Stencil st = P2 ? stencil_texel(a.n, xa, ya, za) : stencil<false>(a.n, tex(a, p.x), tex(a, p.y), tex(a, p.z));
     if (fs < 0 && skippable(a, st, thr, ck, cbk)) return 1;
     f = trilinear_cached<FMT>(a.f, a.nb, st, cc);
     return 0;
@@ -387,7 +397,7 @@ __device__ __forceinline__ RayGeom ray_geom(const RaycastVolume& a, const float*
 // crossing) and its normal -grad F, written by one lane.  The 6 gradient
 // samples' corner loads are issued together; x + 0 == x, so the samples are
 // those of raycast.cu:33-38.
-template <int FMT>
+template <int FMT, bool P2>
 __device__ __forceinline__ void write_surface(const RaycastVolume& a, const RaycastLevel& L, int i, int j,
                                               const RayGeom& g, float tstar)
 {
@@ -398,9 +408,9 @@ __device__ __forceinline__ void write_surface(const RaycastVolume& a, const Rayc
         const float r = a.mue * 0.5f / width;
         const float u = tex(a, vtx.x), v = tex(a, vtx.y), s = tex(a, vtx.z);
         // the 48 corner loads of the 6 samples are issued together (one round trip)
-        const float gx = sample_tsdf<FMT>(a.f, a.n, a.nb, u - r, v, s) - sample_tsdf<FMT>(a.f, a.n, a.nb, u + r, v, s);
-        const float gy = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v - r, s) - sample_tsdf<FMT>(a.f, a.n, a.nb, u, v + r, s);
-        const float gz = sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s - r) - sample_tsdf<FMT>(a.f, a.n, a.nb, u, v, s + r);
+        const float gx = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u - r, v, s) - sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u + r, v, s);
+        const float gy = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u, v - r, s) - sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u, v + r, s);
+        const float gz = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u, v, s - r) - sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u, v, s + r);
         const v3 gg = div3s(mk3(gx, gy, gz), r);
         const float gn = len3(gg);
         if (is_depth_valid(gn)) {
@@ -416,7 +426,7 @@ __device__ __forceinline__ void write_surface(const RaycastVolume& a, const Rayc
 
 // The same, with the 6 gradient samples taken by lanes 0-5 of the wave and the
 // result written by lane 0 (the finishing pass: one ray per wave).
-template <int FMT>
+template <int FMT, bool P2>
 __device__ __forceinline__ void write_surface_wave(const RaycastVolume& a, const RaycastLevel& L, int i, int j,
                                                    const RayGeom& g, float tstar)
 {
@@ -431,8 +441,8 @@ __device__ __forceinline__ void write_surface_wave(const RaycastVolume& a, const
         if (lane < 6) {
             const int ax = lane >> 1;
             const float du = ax == 0 ? r : 0.0f, dv = ax == 1 ? r : 0.0f, ds = ax == 2 ? r : 0.0f;
-            if (lane & 1) val = sample_tsdf<FMT>(a.f, a.n, a.nb, u - du, v - dv, s - ds);
-            else val = sample_tsdf<FMT>(a.f, a.n, a.nb, u + du, v + dv, s + ds);
+            if (lane & 1) val = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u - du, v - dv, s - ds);
+            else val = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, u + du, v + dv, s + ds);
         }
         const float gx = lane_bcast(val, 1) - lane_bcast(val, 0);
         const float gy = lane_bcast(val, 3) - lane_bcast(val, 2);
@@ -468,7 +478,7 @@ struct RayQueue {
 // its exact march state and left to the finishing pass (when q is full it
 // marches on here).  A skipped sample only sets prev_t; its value is computed
 // if and when the zero-crossing formula needs it.
-template <int FMT>
+template <int FMT, bool P2>
 __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const float* pose, const RaycastLevel& L, int i,
                                               int j, bool in_image, const RayQueue& rq, int job)
 {
@@ -519,7 +529,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         // so lanes of a wave in different phases of their march do not run two paths
         const bool lg = step == large;
         float f = 0.0f;
-        if (march_sample<FMT>(a, o, dir, t, width, invw, nf, lg ? kSkipBound : kSmallSkipBound, cs, cb, ck, cbk, cc,
+        if (march_sample<FMT, P2>(a, o, dir, t, width, invw, nf, lg ? kSkipBound : kSmallSkipBound, cs, cb, ck, cbk, cc,
                               f)) {
             // a skipped large step, and the following ones inside the same free
             // super-brick: positions by the same additions, no coordinates computed
@@ -544,7 +554,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
             }
             if (prev_skipped) {
                 const v3 pp = add3(o, mul3s(dir, prev_t));
-                prev = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
+                prev = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
             }
             tstar = t - small * prev / (f - prev);
             break;
@@ -554,7 +564,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastVolume& a, const floa
         prev_skipped = false;
         t = t + step;
     }
-    write_surface<FMT>(a, L, i, j, g, tstar);
+    write_surface<FMT, P2>(a, L, i, j, g, tstar);
 }
 
 // Position of sample `lane` of a chunk of 64: t_0 = t and t_{k+1} = t_k + step,
@@ -594,7 +604,7 @@ __device__ __forceinline__ float chunk_position(float t, float step, int lane)
 // single-lane march would (prev from the last of them), the ones after it are
 // discarded, so every sample position, value and decision is the sequential
 // march's and the output is the same.
-template <int FMT>
+template <int FMT, bool P2>
 __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* pose, const RaycastLevel& L,
                                            const DeferredRay& r)
 {
@@ -618,7 +628,7 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
         float f = 0.0f;
         int sk = 1;
         if (!end)
-            sk = march_sample<FMT>(a, g.o, g.dir, tk, width, invw, nf, step == large ? kSkipBound : kSmallSkipBound,
+            sk = march_sample<FMT, P2>(a, g.o, g.dir, tk, width, invw, nf, step == large ? kSkipBound : kSmallSkipBound,
                                    cs, cb, ck, cbk, cc, f);
         const bool neg = !end && !sk && f < 0.0f;
         const bool slow = !end && !sk && step == large && f < 0.99f;
@@ -653,7 +663,7 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
             float pk = pv;
             if (psk) {
                 const v3 pp = add3(g.o, mul3s(g.dir, pt));
-                pk = sample_tsdf<FMT>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
+                pk = sample_tsdf<FMT, P2>(a.f, a.n, a.nb, tex(a, pp.x), tex(a, pp.y), tex(a, pp.z));
             }
             tstar = tkk - step * pk / (fk - pk);
             break;
@@ -665,7 +675,7 @@ __device__ __forceinline__ void finish_ray(const RaycastVolume& a, const float* 
         t = tkk + step;
         gb = gb + k + 1;
     }
-    write_surface_wave<FMT>(a, L, i, j, g, tstar);
+    write_surface_wave<FMT, P2>(a, L, i, j, g, tstar);
 }
 
 // Workgroups of the first pass per level: its tile count rounded up to 8.
@@ -740,7 +750,9 @@ __global__ __launch_bounds__(64, 5) void raycast_kernel(RaycastArgs a)
         const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
         const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
         const RayQueue rq{a.rays, a.count, a.cap, kLaneBudget};
-        raycast_pixel<FMT>(a.vol, a.dyn ? a.dyn->pose : a.pose, L, i, j, i < L.w && j < L.h, rq, l);
+        const float* pose = a.dyn ? a.dyn->pose : a.pose;
+        if (a.vol.p2) raycast_pixel<FMT, true>(a.vol, pose, L, i, j, i < L.w && j < L.h, rq, l);
+        else raycast_pixel<FMT, false>(a.vol, pose, L, i, j, i < L.w && j < L.h, rq, l);
     }
     if (a.order_out && threadIdx.x == 0)
         record_order(a, l, x, per, base, slot, live ? wall_clock64() - t0 : 0ull);
@@ -778,7 +790,8 @@ __global__ __launch_bounds__(64 * kFinishPer) void raycast_finish_kernel(Raycast
         d.t = cr.t;
         d.prev = cr.prev;
         d.prev_t = cr.prev_t;
-        finish_ray<FMT>(a.vol, pose, a.lv[d.job], d);
+        if (a.vol.p2) finish_ray<FMT, true>(a.vol, pose, a.lv[d.job], d);
+        else finish_ray<FMT, false>(a.vol, pose, a.lv[d.job], d);
     }
 }
 
@@ -796,6 +809,7 @@ __device__ __forceinline__ void copy_job(const CJ& j, RaycastVolume& vol, Raycas
     vol.width = j.vol.width;
     vol.mue = j.vol.mue;
     vol.inv_width = j.vol.inv_width;
+    vol.p2 = j.vol.p2;
 #pragma unroll
     for (int q = 0; q < 9; ++q) L.Kinv[q] = j.L.Kinv[q];
     L.vout = j.L.vout;
@@ -856,7 +870,8 @@ __global__ __launch_bounds__(64) void raycast_batch_kernel(const BatchHeader* hd
                 const int lt = t - L.tile0;
                 const int i = (lt % L.tiles_x) * 8 + (threadIdx.x & 7);
                 const int j = (lt / L.tiles_x) * 8 + (threadIdx.x >> 3);
-                raycast_pixel<FMT>(vol, pose, L, i, j, i < L.w && j < L.h, rq, jb);
+                if (vol.p2) raycast_pixel<FMT, true>(vol, pose, L, i, j, i < L.w && j < L.h, rq, jb);
+                else raycast_pixel<FMT, false>(vol, pose, L, i, j, i < L.w && j < L.h, rq, jb);
             }
         }
     }
@@ -891,7 +906,8 @@ __global__ __launch_bounds__(64 * kFinishPer) void raycast_batch_finish_kernel(c
         RaycastVolume vol;
         RaycastLevel L;
         copy_job(cj[d.job], vol, L);
-        finish_ray<FMT>(vol, cj[d.job].pose, L, d);
+        if (vol.p2) finish_ray<FMT, true>(vol, cj[d.job].pose, L, d);
+        else finish_ray<FMT, false>(vol, cj[d.job].pose, L, d);
     }
 }
 
@@ -911,6 +927,7 @@ RaycastVolume raycast_volume(const kf_volume* v)
     r.mue = v->mue;
     int e = 0;
     r.inv_width = frexpf(v->width, &e) == 0.5f ? 1.0f / v->width : 0.0f;
+    r.p2 = r.inv_width != 0.0f && (v->n & (v->n - 1)) == 0 ? 1 : 0;
     return r;
 }
 
